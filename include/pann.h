@@ -1,0 +1,164 @@
+/*
+ * pann.h -- C ABI of the MI355X (gfx950) graph beam search.
+ *
+ * Drop-in for the reference's search path: the C++ structs VectoresGPU/GrafoGPU
+ * (cuda/datos_gpu.h:8-28), HnswGPU (cuda/hnsw.h:43-58), BuscadorGPU
+ * (cuda/busqueda.cuh:36-62) and BuscadorHnsw (cuda/hnsw.h:65-86).  Each of their
+ * methods maps to one entry point below (cited per function).  Differences that are
+ * part of the boundary:
+ *   - errors return a status code and set pann_last_error() instead of exit(1)
+ *     (cuda/comun.h:57-65, cuda/busqueda.cu:530-534);
+ *   - the query index used by ParlayANN's "a == p.id()" exclusion can be given
+ *     explicitly per query (the reference always uses the row index);
+ *   - per-query statistics include adjacency entries read and hops next to the
+ *     distance count and flags.
+ * All device pointers are HIP device memory; `stream` is a hipStream_t (NULL = the
+ * default stream).  Launch functions are asynchronous on that stream.
+ */
+#ifndef PANN_H
+#define PANN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes and flags ---- */
+#define PANN_OK 0
+#define PANN_ERR_ARG 1      /* invalid argument */
+#define PANN_ERR_IO 2       /* file missing or unreadable */
+#define PANN_ERR_FORMAT 3   /* file contents do not match the format */
+#define PANN_ERR_HIP 4      /* HIP runtime error (no device, launch failure, ...) */
+#define PANN_ERR_CAPACITY 5 /* size beyond what the kernels support */
+
+#define PANN_NO_NEIGHBOR 0xFFFFFFFFu /* SIN_VECINO, cuda/comun.h:22 */
+#define PANN_FLAG_OVERFLOW 1         /* BANDERA_DESBORDE, cuda/busqueda.cuh:14 */
+#define PANN_FLAG_SHORT 2            /* BANDERA_FRONTERA_CORTA, cuda/busqueda.cuh:15 */
+#define PANN_FLAG_CONFIG 4           /* query parameters outside what the searcher was sized for */
+
+typedef struct pann_vectors pann_vectors;   /* VectoresGPU */
+typedef struct pann_graph pann_graph;       /* GrafoGPU */
+typedef struct pann_hnsw pann_hnsw;         /* HnswGPU */
+typedef struct pann_searcher pann_searcher; /* BuscadorGPU / BuscadorHnsw */
+
+/* ConfigBusqueda (cuda/busqueda.cuh:6-11).  visit_limit <= 0 means "no visit
+ * argument" (n).  Defaults: Ls 10, k 10, cut 1.35. */
+typedef struct pann_search_config {
+    int32_t Ls;
+    int32_t k;
+    int64_t visit_limit;
+    double cut;
+} pann_search_config;
+
+/* One query of the batched form: its own qid, Ls, k, visit limit and cut. */
+typedef struct pann_query_params {
+    uint32_t qid;
+    int32_t Ls;
+    int32_t k;
+    int32_t reserved;
+    int64_t visit_limit;
+    double cut;
+} pann_query_params;
+
+/* Device output buffers owned by a searcher (idsDev, distsDev, compDev, banderasDev,
+ * cuda/busqueda.cuh:42-46) plus adjacency entries read and hops per query. */
+typedef struct pann_search_outputs {
+    uint32_t* ids;    /* nq x stride */
+    int32_t* dists;   /* nq x stride */
+    int32_t* comps;   /* nq: distance computations (ParlayANN dist_cmps) */
+    int32_t* flags;   /* nq: PANN_FLAG_* */
+    int32_t* edges;   /* nq: adjacency entries read, padding excluded */
+    int32_t* hops;    /* nq: nodes expanded */
+    uint32_t stride;  /* = k of the searcher */
+    uint32_t nq;
+} pann_search_outputs;
+
+/* ---- library ---- */
+const char* pann_last_error(void); /* message of the last failure on this thread */
+int pann_version(void);            /* 10000 * major + 100 * minor + patch */
+int pann_device_count(int* count); /* PANN_ERR_HIP when no HIP device is usable */
+int pann_synchronize(void* stream);   /* hipStreamSynchronize */
+int pann_device_synchronize(void);    /* hipDeviceSynchronize */
+
+/* ---- device memory, streams and events for callers without their own HIP layer ---- */
+int pann_device_alloc(size_t bytes, void** out);
+void pann_device_free(void* p);
+int pann_host_alloc(size_t bytes, void** out); /* pinned host memory */
+void pann_host_free(void* p);
+/* hipMemcpyAsync with hipMemcpyDefault (direction from the pointers); blocking when
+ * stream is NULL. */
+int pann_memcpy(void* dst, const void* src, size_t bytes, void* stream);
+int pann_stream_create(void** out);
+void pann_stream_destroy(void* s);
+int pann_event_create(void** out);
+int pann_event_record(void* ev, void* stream);
+int pann_event_elapsed_ms(void* start, void* end, float* ms); /* waits for `end` */
+void pann_event_destroy(void* ev);
+
+/* ---- vectors (VectoresGPU) ---- */
+/* subirVectores (cuda/datos_gpu.cu:20-40): host rows of `dim` int8 -> device rows with a
+ * 128-byte pitch, zero padded.  dim <= 128. */
+int pann_vectors_create(const int8_t* host_rows, uint32_t n, uint32_t dim, pann_vectors** out);
+/* leerVectoresI8bin (cuda/lectores.cpp:70-95) followed by subirVectores. */
+int pann_vectors_load(const char* i8bin_path, pann_vectors** out);
+/* copiarVectoresAsync (cuda/datos_gpu.cu:48-56): overwrite the first n rows from host
+ * rows of `dim` bytes (pinned memory gives an asynchronous copy). */
+int pann_vectors_copy_async(pann_vectors* v, const int8_t* host_rows, uint32_t n, void* stream);
+int pann_vectors_info(const pann_vectors* v, uint32_t* n, uint32_t* dim, const int8_t** device_rows,
+                      uint32_t* pitch);
+void pann_vectors_free(pann_vectors* v); /* liberarVectores */
+
+/* ---- flat graph (GrafoGPU) ---- */
+/* subirGrafo (cuda/datos_gpu.cu:64-81): ParlayANN CSR (degrees[n], edges concatenated)
+ * -> dense n x max_degree matrix padded with PANN_NO_NEIGHBOR. */
+int pann_graph_create(uint32_t n, uint32_t max_degree, const uint32_t* degrees,
+                      const uint32_t* edges, pann_graph** out);
+/* leerGrafoParlay (cuda/lectores.cpp:97-144) followed by subirGrafo. */
+int pann_graph_load(const char* path, pann_graph** out);
+int pann_graph_info(const pann_graph* g, uint32_t* n, uint32_t* max_degree);
+void pann_graph_free(pann_graph* g); /* liberarGrafo */
+
+/* ---- HNSW (HnswGPU) ---- */
+/* leerHnsw (cuda/hnsw_lector.cpp:28-111) + subirHnsw (cuda/busqueda_hnsw.cu:462-530).
+ * `base` holds the base vectors indexed by point id. */
+int pann_hnsw_load(const char* path, const pann_vectors* base, pann_hnsw** out);
+/* Same from host arrays in the file's per-node layout: level[n], node_point[n]; for node i
+ * and layer l <= level[i], list i,l is list_ids[list_offsets[j] .. + list_sizes[j]) with
+ * j = node_first_list[i] + l; entries[num_entries]. */
+int pann_hnsw_create(uint32_t n, const uint32_t* level, const uint32_t* node_point,
+                     const uint64_t* node_first_list, const uint64_t* list_offsets,
+                     const uint32_t* list_sizes, const uint32_t* list_ids,
+                     const uint32_t* entries, uint32_t num_entries, const pann_vectors* base,
+                     pann_hnsw** out);
+int pann_hnsw_info(const pann_hnsw* h, uint32_t* n, uint32_t* max_degree0,
+                   uint32_t* max_degree_upper, uint32_t* top_level, uint32_t* num_entries);
+void pann_hnsw_free(pann_hnsw* h); /* liberarHnsw */
+
+/* ---- plain searcher: one configuration for a block of nq queries ---- */
+/* BuscadorGPU::preparar (cuda/busqueda.cu:525-568): sizes LDS/scratch and allocates the
+ * outputs for nq queries of graph g under cfg. */
+int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_config* cfg,
+                          pann_searcher** out);
+/* BuscadorGPU::lanzar (cuda/busqueda.cu:570-577).  queries: nq rows.  qids: optional
+ * device array of nq query ids (NULL: the row index, as the reference). */
+int pann_searcher_launch(pann_searcher* s, const pann_vectors* base, const pann_vectors* queries,
+                         const pann_graph* g, const uint32_t* qids, void* stream);
+/* BuscadorHnsw::preparar / lanzar (cuda/busqueda_hnsw.cu:551-630). */
+int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_search_config* cfg,
+                               pann_searcher** out);
+int pann_hnsw_searcher_launch(pann_searcher* s, const pann_vectors* queries, const pann_hnsw* h,
+                              void* stream);
+int pann_searcher_outputs(const pann_searcher* s, pann_search_outputs* out);
+/* The D2H copies of the reference's end-to-end path (cuda/comparacion1.cu:68-74): copies
+ * the outputs of the last launch into host arrays (any may be NULL) on `stream`;
+ * blocking when stream is NULL. */
+int pann_searcher_download(const pann_searcher* s, uint32_t* ids, int32_t* dists, int32_t* comps,
+                           int32_t* flags, int32_t* edges, int32_t* hops, void* stream);
+void pann_searcher_free(pann_searcher* s); /* liberar */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PANN_H */

@@ -1,0 +1,596 @@
+// pann_api.cpp -- the C ABI of include/pann.h: device-resident vectors and graphs, and
+// searchers that size LDS/scratch and launch the gfx950 kernels.
+#include "../../include/pann.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "formats.h"
+#include "pann_internal.h"
+
+using namespace pann;
+
+struct pann_vectors {
+    uint32_t n = 0, dim = 0;
+    int8_t* dev = nullptr;  // n x kVecBytes
+};
+
+struct pann_graph {
+    uint32_t n = 0, G = 0;
+    uint32_t* adj = nullptr;  // n x G
+};
+
+struct pann_hnsw {
+    uint32_t n = 0;
+    uint32_t G0 = 0, GA = 0;
+    uint32_t top = 0, numEntries = 0;
+    uint32_t entries[8] = {};
+    uint32_t* adj0 = nullptr;       // n x G0
+    uint32_t* adjUp = nullptr;      // blocks x GA
+    uint32_t* upStart = nullptr;    // n
+    uint32_t* nodePoint = nullptr;  // n
+    int8_t* nodeVec = nullptr;      // n x kVecBytes, vectors in node order
+    bool ownVec = false;
+    uint32_t dim = 0;
+};
+
+struct pann_searcher {
+    int kind = 0;  // 0 flat, 1 hnsw
+    pann_search_config cfg{};
+    uint32_t nq = 0;
+    int maxch = 1;
+    int wavesPerBlock = 4;
+    int blocks = 0;
+    FlatLaunch L{};
+    uint32_t* gfilt = nullptr;
+    pann_search_outputs out{};
+};
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int setError(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_lastError = buf;
+    return code;
+}
+
+int fromStatus(const Status& s) {
+    if (s.ok()) return PANN_OK;
+    g_lastError = s.msg;
+    return s.code;
+}
+
+#define HIP_TRY(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return setError(PANN_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+int deviceCUs(int* cus) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return PANN_OK;
+}
+
+// rows of `dim` bytes -> device rows with kVecBytes pitch (padding stays zero)
+int uploadRows(int8_t* dev, const int8_t* host, uint32_t n, uint32_t dim, hipStream_t s,
+               bool async) {
+    if (n == 0) return PANN_OK;
+    if (async) {
+        HIP_TRY(hipMemcpy2DAsync(dev, kVecBytes, host, dim, dim, n, hipMemcpyHostToDevice, s));
+    } else {
+        HIP_TRY(hipMemcpy2D(dev, kVecBytes, host, dim, dim, n, hipMemcpyHostToDevice));
+    }
+    return PANN_OK;
+}
+
+void freeOutputs(pann_search_outputs& o) {
+    hipFree(o.ids);
+    hipFree(o.dists);
+    hipFree(o.comps);
+    hipFree(o.flags);
+    hipFree(o.edges);
+    hipFree(o.hops);
+    o = pann_search_outputs{};
+}
+
+int allocOutputs(pann_search_outputs& o, uint32_t nq, uint32_t stride) {
+    o.nq = nq;
+    o.stride = stride;
+    const size_t nk = std::max<size_t>(1, (size_t)nq * stride), nn = std::max<uint32_t>(1, nq);
+    HIP_TRY(hipMalloc(&o.ids, nk * 4));
+    HIP_TRY(hipMalloc(&o.dists, nk * 4));
+    HIP_TRY(hipMalloc(&o.comps, nn * 4));
+    HIP_TRY(hipMalloc(&o.flags, nn * 4));
+    HIP_TRY(hipMalloc(&o.edges, nn * 4));
+    HIP_TRY(hipMalloc(&o.hops, nn * 4));
+    HIP_TRY(hipMemset(o.flags, 0, nn * 4));
+    return PANN_OK;
+}
+
+int checkConfig(const pann_search_config* cfg) {
+    if (!cfg) return setError(PANN_ERR_ARG, "config is NULL");
+    if (cfg->Ls < 1) return setError(PANN_ERR_ARG, "Ls must be >= 1 (got %d)", cfg->Ls);
+    if (cfg->k < 0) return setError(PANN_ERR_ARG, "k must be >= 0 (got %d)", cfg->k);
+    if (!(cfg->cut > 0)) return setError(PANN_ERR_ARG, "cut must be > 0");
+    return PANN_OK;
+}
+
+// Largest filter kept in LDS (2^12 u32 = 16 KB); larger ones use a per-wave HBM region.
+constexpr int kFilterBitsLds = 12;
+constexpr size_t kMaxLdsPerBlock = 64 * 1024;
+
+}  // namespace
+
+extern "C" {
+
+const char* pann_last_error(void) { return g_lastError.c_str(); }
+
+int pann_version(void) { return 100; }
+
+int pann_device_count(int* count) {
+    if (!count) return setError(PANN_ERR_ARG, "count is NULL");
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess || *count == 0) {
+        *count = 0;
+        return setError(PANN_ERR_HIP, "no HIP device: %s", hipGetErrorString(e));
+    }
+    return PANN_OK;
+}
+
+int pann_synchronize(void* stream) {
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return PANN_OK;
+}
+
+int pann_device_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return PANN_OK;
+}
+
+int pann_device_alloc(size_t bytes, void** out) {
+    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    HIP_TRY(hipMalloc(out, std::max<size_t>(bytes, 16)));
+    return PANN_OK;
+}
+
+void pann_device_free(void* p) { hipFree(p); }
+
+int pann_host_alloc(size_t bytes, void** out) {
+    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    HIP_TRY(hipHostMalloc(out, std::max<size_t>(bytes, 16), hipHostMallocDefault));
+    return PANN_OK;
+}
+
+void pann_host_free(void* p) { hipHostFree(p); }
+
+int pann_memcpy(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return PANN_OK;
+    if (!dst || !src) return setError(PANN_ERR_ARG, "NULL pointer");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, (hipStream_t)stream));
+    if (!stream) HIP_TRY(hipStreamSynchronize(nullptr));
+    return PANN_OK;
+}
+
+int pann_stream_create(void** out) {
+    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
+    HIP_TRY(hipStreamCreateWithFlags((hipStream_t*)out, hipStreamNonBlocking));
+    return PANN_OK;
+}
+
+void pann_stream_destroy(void* s) {
+    if (s) hipStreamDestroy((hipStream_t)s);
+}
+
+int pann_event_create(void** out) {
+    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
+    HIP_TRY(hipEventCreate((hipEvent_t*)out));
+    return PANN_OK;
+}
+
+int pann_event_record(void* ev, void* stream) {
+    HIP_TRY(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return PANN_OK;
+}
+
+int pann_event_elapsed_ms(void* start, void* end, float* ms) {
+    if (!ms) return setError(PANN_ERR_ARG, "ms is NULL");
+    HIP_TRY(hipEventSynchronize((hipEvent_t)end));
+    HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+    return PANN_OK;
+}
+
+void pann_event_destroy(void* ev) {
+    if (ev) hipEventDestroy((hipEvent_t)ev);
+}
+
+// ---------------------------------------------------------------- vectors
+int pann_vectors_create(const int8_t* host_rows, uint32_t n, uint32_t dim, pann_vectors** out) {
+    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (dim == 0 || dim > (uint32_t)kMaxDim)
+        return setError(PANN_ERR_CAPACITY, "dim %u not supported (1..%d)", dim, kMaxDim);
+    if (n > 0 && !host_rows) return setError(PANN_ERR_ARG, "host_rows is NULL");
+    pann_vectors* v = new pann_vectors();
+    v->n = n;
+    v->dim = dim;
+    const size_t bytes = std::max<size_t>(kVecBytes, (size_t)n * kVecBytes);
+    hipError_t e = hipMalloc(&v->dev, bytes);
+    if (e == hipSuccess) e = hipMemset(v->dev, 0, bytes);
+    if (e != hipSuccess) {
+        hipFree(v->dev);
+        delete v;
+        return setError(PANN_ERR_HIP, "allocating %zu bytes of vectors: %s", bytes, hipGetErrorString(e));
+    }
+    int r = uploadRows(v->dev, host_rows, n, dim, nullptr, false);
+    if (r != PANN_OK) {
+        pann_vectors_free(v);
+        return r;
+    }
+    *out = v;
+    return PANN_OK;
+}
+
+int pann_vectors_load(const char* path, pann_vectors** out) {
+    if (!path || !out) return setError(PANN_ERR_ARG, "NULL argument");
+    HostVectors hv;
+    int r = fromStatus(readI8bin(path, hv));
+    if (r != PANN_OK) return r;
+    return pann_vectors_create(hv.data.data(), hv.n, hv.dim, out);
+}
+
+int pann_vectors_copy_async(pann_vectors* v, const int8_t* host_rows, uint32_t n, void* stream) {
+    if (!v || (!host_rows && n)) return setError(PANN_ERR_ARG, "NULL argument");
+    if (n > v->n) return setError(PANN_ERR_ARG, "%u rows do not fit in %u", n, v->n);
+    return uploadRows(v->dev, host_rows, n, v->dim, (hipStream_t)stream, true);
+}
+
+int pann_vectors_info(const pann_vectors* v, uint32_t* n, uint32_t* dim, const int8_t** rows,
+                      uint32_t* pitch) {
+    if (!v) return setError(PANN_ERR_ARG, "vectors is NULL");
+    if (n) *n = v->n;
+    if (dim) *dim = v->dim;
+    if (rows) *rows = v->dev;
+    if (pitch) *pitch = kVecBytes;
+    return PANN_OK;
+}
+
+void pann_vectors_free(pann_vectors* v) {
+    if (!v) return;
+    hipFree(v->dev);
+    delete v;
+}
+
+// ---------------------------------------------------------------- flat graph
+int pann_graph_create(uint32_t n, uint32_t max_degree, const uint32_t* degrees,
+                      const uint32_t* edges, pann_graph** out) {
+    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (n == 0 || !degrees) return setError(PANN_ERR_ARG, "empty graph");
+    uint64_t total = 0;
+    int r = fromStatus(checkGraph(n, max_degree, degrees, edges, &total));
+    if (r != PANN_OK) return r;
+    if (total > 0 && !edges) return setError(PANN_ERR_ARG, "edges is NULL");
+    if (max_degree == 0) max_degree = 1;
+    if (max_degree > 256)
+        return setError(PANN_ERR_CAPACITY, "max degree %u above the supported 256", max_degree);
+    std::vector<uint32_t> dense((size_t)n * max_degree, kEmpty);
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        memcpy(dense.data() + (size_t)i * max_degree, edges + off, 4ull * degrees[i]);
+        off += degrees[i];
+    }
+    pann_graph* g = new pann_graph();
+    g->n = n;
+    g->G = max_degree;
+    hipError_t e = hipMalloc(&g->adj, dense.size() * 4);
+    if (e == hipSuccess) e = hipMemcpy(g->adj, dense.data(), dense.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(g->adj);
+        delete g;
+        return setError(PANN_ERR_HIP, "uploading graph: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return PANN_OK;
+}
+
+int pann_graph_load(const char* path, pann_graph** out) {
+    if (!path || !out) return setError(PANN_ERR_ARG, "NULL argument");
+    HostGraph hg;
+    int r = fromStatus(readParlayGraph(path, hg));
+    if (r != PANN_OK) return r;
+    return pann_graph_create(hg.n, hg.maxDeg, hg.degrees.data(), hg.edges.data(), out);
+}
+
+int pann_graph_info(const pann_graph* g, uint32_t* n, uint32_t* max_degree) {
+    if (!g) return setError(PANN_ERR_ARG, "graph is NULL");
+    if (n) *n = g->n;
+    if (max_degree) *max_degree = g->G;
+    return PANN_OK;
+}
+
+void pann_graph_free(pann_graph* g) {
+    if (!g) return;
+    hipFree(g->adj);
+    delete g;
+}
+
+// ---------------------------------------------------------------- HNSW
+int pann_hnsw_create(uint32_t n, const uint32_t* level, const uint32_t* node_point,
+                     const uint64_t* node_first_list, const uint64_t* list_offsets,
+                     const uint32_t* list_sizes, const uint32_t* list_ids,
+                     const uint32_t* entries, uint32_t num_entries, const pann_vectors* base,
+                     pann_hnsw** out) {
+    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!n || !level || !node_point || !node_first_list || !list_offsets || !list_sizes ||
+        !entries || !base)
+        return setError(PANN_ERR_ARG, "NULL argument");
+    if (num_entries < 1 || num_entries > 8)
+        return setError(PANN_ERR_FORMAT, "%u entry points (1..8 supported)", num_entries);
+    uint32_t G0 = 1, GA = 1;
+    uint64_t blocks = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        if (node_point[i] >= base->n)
+            return setError(PANN_ERR_FORMAT, "node %u holds point %u beyond the base (%u)", i,
+                            node_point[i], base->n);
+        for (uint32_t l = 0; l <= level[i]; l++) {
+            const uint32_t sz = list_sizes[node_first_list[i] + l];
+            if (l == 0) G0 = std::max(G0, sz);
+            else GA = std::max(GA, sz);
+            const uint32_t* ids = list_ids + list_offsets[node_first_list[i] + l];
+            for (uint32_t j = 0; j < sz; j++)
+                if (ids[j] >= n) return setError(PANN_ERR_FORMAT, "neighbour %u out of range", ids[j]);
+        }
+        blocks += level[i];
+    }
+    for (uint32_t e = 0; e < num_entries; e++)
+        if (entries[e] >= n) return setError(PANN_ERR_FORMAT, "entry point out of range");
+    if (G0 > 256 || GA > 256)
+        return setError(PANN_ERR_CAPACITY, "degree %u/%u above the supported 256", G0, GA);
+
+    std::vector<uint32_t> adj0((size_t)n * G0, kEmpty);
+    std::vector<uint32_t> up(std::max<uint64_t>(1, blocks) * GA, kEmpty);
+    std::vector<uint32_t> start(n, kEmpty);
+    uint64_t b = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint64_t f = node_first_list[i];
+        memcpy(adj0.data() + (size_t)i * G0, list_ids + list_offsets[f], 4ull * list_sizes[f]);
+        if (level[i] >= 1) {
+            start[i] = (uint32_t)b;
+            for (uint32_t l = 1; l <= level[i]; l++)
+                memcpy(up.data() + (b + l - 1) * GA, list_ids + list_offsets[f + l],
+                       4ull * list_sizes[f + l]);
+            b += level[i];
+        }
+    }
+    pann_hnsw* h = new pann_hnsw();
+    h->n = n;
+    h->G0 = G0;
+    h->GA = GA;
+    h->numEntries = num_entries;
+    for (uint32_t e = 0; e < num_entries; e++) h->entries[e] = entries[e];
+    h->top = level[entries[0]];
+    h->dim = base->dim;
+    bool identity = base->n == n;
+    for (uint32_t i = 0; i < n && identity; i++) identity = node_point[i] == i;
+    hipError_t e = hipMalloc(&h->adj0, adj0.size() * 4);
+    if (e == hipSuccess) e = hipMemcpy(h->adj0, adj0.data(), adj0.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&h->adjUp, up.size() * 4);
+    if (e == hipSuccess) e = hipMemcpy(h->adjUp, up.data(), up.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&h->upStart, 4ull * n);
+    if (e == hipSuccess) e = hipMemcpy(h->upStart, start.data(), 4ull * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&h->nodePoint, 4ull * n);
+    if (e == hipSuccess) e = hipMemcpy(h->nodePoint, node_point, 4ull * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        if (identity) {
+            h->nodeVec = base->dev;
+            h->ownVec = false;
+        } else {
+            // vectors in node order, copied device to device
+            e = hipMalloc(&h->nodeVec, (size_t)n * kVecBytes);
+            h->ownVec = e == hipSuccess;
+            for (uint32_t i = 0; e == hipSuccess && i < n; i++)
+                e = hipMemcpyAsync(h->nodeVec + (size_t)i * kVecBytes,
+                                   base->dev + (size_t)node_point[i] * kVecBytes, kVecBytes,
+                                   hipMemcpyDeviceToDevice, nullptr);
+            if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+        }
+    }
+    if (e != hipSuccess) {
+        pann_hnsw_free(h);
+        return setError(PANN_ERR_HIP, "uploading HNSW: %s", hipGetErrorString(e));
+    }
+    *out = h;
+    return PANN_OK;
+}
+
+int pann_hnsw_load(const char* path, const pann_vectors* base, pann_hnsw** out) {
+    if (!path || !base || !out) return setError(PANN_ERR_ARG, "NULL argument");
+    HostHnsw h;
+    int r = fromStatus(readHnsw(path, h));
+    if (r != PANN_OK) return r;
+    return pann_hnsw_create(h.n, h.level.data(), h.nodePoint.data(), h.firstList.data(),
+                            h.listOffset.data(), h.listSize.data(), h.ids.data(),
+                            h.entries.data(), (uint32_t)h.entries.size(), base, out);
+}
+
+int pann_hnsw_info(const pann_hnsw* h, uint32_t* n, uint32_t* g0, uint32_t* ga, uint32_t* top,
+                   uint32_t* ne) {
+    if (!h) return setError(PANN_ERR_ARG, "hnsw is NULL");
+    if (n) *n = h->n;
+    if (g0) *g0 = h->G0;
+    if (ga) *ga = h->GA;
+    if (top) *top = h->top;
+    if (ne) *ne = h->numEntries;
+    return PANN_OK;
+}
+
+void pann_hnsw_free(pann_hnsw* h) {
+    if (!h) return;
+    hipFree(h->adj0);
+    hipFree(h->adjUp);
+    hipFree(h->upStart);
+    hipFree(h->nodePoint);
+    if (h->ownVec) hipFree(h->nodeVec);
+    delete h;
+}
+
+// ---------------------------------------------------------------- flat searcher
+int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_config* cfg,
+                          pann_searcher** out) {
+    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!g) return setError(PANN_ERR_ARG, "graph is NULL");
+    int r = checkConfig(cfg);
+    if (r != PANN_OK) return r;
+    const int maxch = g->G <= 64 ? 1 : g->G <= 128 ? 2 : 4;
+    const int Ls = cfg->Ls;
+    const int fbits = flatFilterBits(Ls);
+    const bool ldsFilter = fbits <= kFilterBitsLds;
+    const uint32_t capV = flatVisitedCap(Ls);
+    const FlatLds lay = flatLdsLayout(Ls, maxch, capV, ldsFilter ? fbits : 0);
+    if (lay.total > kMaxLdsPerBlock)
+        return setError(PANN_ERR_CAPACITY, "Ls=%d needs %u bytes of LDS per query", Ls, lay.total);
+
+    pann_searcher* s = new pann_searcher();
+    s->kind = 0;
+    s->cfg = *cfg;
+    s->nq = nq;
+    s->maxch = maxch;
+    s->wavesPerBlock = (int)std::min<size_t>(4, std::max<size_t>(1, kMaxLdsPerBlock / lay.total));
+    int bpc = 0, cus = 0;
+    hipError_t he = beamFlatOccupancy(maxch, s->wavesPerBlock, (size_t)lay.total * s->wavesPerBlock, &bpc);
+    r = deviceCUs(&cus);
+    if (he != hipSuccess || r != PANN_OK) {
+        delete s;
+        return he != hipSuccess ? setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(he)) : r;
+    }
+    const uint32_t want = (nq + s->wavesPerBlock - 1) / s->wavesPerBlock;
+    s->blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>(want, (uint32_t)std::max(1, bpc) * cus));
+    const uint32_t totalWaves = (uint32_t)s->blocks * s->wavesPerBlock;
+
+    FlatLaunch& L = s->L;
+    L.n = g->n;
+    L.G = g->G;
+    L.nq = nq;
+    L.Ls = Ls;
+    L.k = cfg->k;
+    L.limit = cfg->visit_limit > 0 ? cfg->visit_limit : (int64_t)g->n;
+    L.cut = cfg->cut;
+    L.LsMax = Ls;
+    L.fbitsLds = ldsFilter ? fbits : 0;
+    L.gbitsMax = ldsFilter ? 0 : fbits;
+    L.capV = capV;
+    L.ldsPerWave = lay.total;
+    L.lds = lay;
+    L.totalWaves = totalWaves;
+    L.queue = nullptr;
+    if (!ldsFilter) {
+        he = hipMalloc(&s->gfilt, ((size_t)totalWaves << fbits) * 4);
+        if (he != hipSuccess) {
+            delete s;
+            return setError(PANN_ERR_HIP, "allocating filter scratch: %s", hipGetErrorString(he));
+        }
+    }
+    L.gfilt = s->gfilt;
+    r = allocOutputs(s->out, nq, (uint32_t)std::max(1, cfg->k));
+    if (r != PANN_OK) {
+        pann_searcher_free(s);
+        return r;
+    }
+    L.outIds = s->out.ids;
+    L.outDists = s->out.dists;
+    L.outStride = s->out.stride;
+    L.outComps = s->out.comps;
+    L.outFlags = s->out.flags;
+    L.outEdges = s->out.edges;
+    L.outHops = s->out.hops;
+    *out = s;
+    return PANN_OK;
+}
+
+int pann_searcher_launch(pann_searcher* s, const pann_vectors* base, const pann_vectors* queries,
+                         const pann_graph* g, const uint32_t* qids, void* stream) {
+    if (!s || !base || !queries || !g) return setError(PANN_ERR_ARG, "NULL argument");
+    if (s->kind != 0) return setError(PANN_ERR_ARG, "searcher was prepared for HNSW");
+    if (base->n != g->n) return setError(PANN_ERR_ARG, "base has %u rows, graph %u nodes", base->n, g->n);
+    if (g->G != s->L.G) return setError(PANN_ERR_ARG, "graph differs from the prepared one");
+    if (base->dim != queries->dim)
+        return setError(PANN_ERR_ARG, "dim mismatch base %u queries %u", base->dim, queries->dim);
+    if (queries->n < s->nq) return setError(PANN_ERR_ARG, "%u queries, searcher prepared for %u", queries->n, s->nq);
+    if (s->nq == 0) return PANN_OK;
+    FlatLaunch L = s->L;
+    L.base = base->dev;
+    L.adj = g->adj;
+    L.queries = queries->dev;
+    L.qids = qids;
+    L.qp = nullptr;
+    hipError_t e = launchBeamFlat(L, s->maxch, s->wavesPerBlock, s->blocks, (hipStream_t)stream);
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "beam search launch: %s", hipGetErrorString(e));
+    return PANN_OK;
+}
+
+int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_search_config* cfg,
+                               pann_searcher** out) {
+    (void)h;
+    (void)nq;
+    (void)cfg;
+    if (out) *out = nullptr;
+    return setError(PANN_ERR_CAPACITY, "HNSW search kernel not available in this build");
+}
+
+int pann_hnsw_searcher_launch(pann_searcher* s, const pann_vectors* queries, const pann_hnsw* h,
+                              void* stream) {
+    (void)s;
+    (void)queries;
+    (void)h;
+    (void)stream;
+    return setError(PANN_ERR_CAPACITY, "HNSW search kernel not available in this build");
+}
+
+int pann_searcher_outputs(const pann_searcher* s, pann_search_outputs* out) {
+    if (!s || !out) return setError(PANN_ERR_ARG, "NULL argument");
+    *out = s->out;
+    return PANN_OK;
+}
+
+int pann_searcher_download(const pann_searcher* s, uint32_t* ids, int32_t* dists, int32_t* comps,
+                           int32_t* flags, int32_t* edges, int32_t* hops, void* stream) {
+    if (!s) return setError(PANN_ERR_ARG, "searcher is NULL");
+    const hipStream_t st = (hipStream_t)stream;
+    const size_t nk = (size_t)s->out.nq * s->out.stride * 4, nn = (size_t)s->out.nq * 4;
+    if (s->out.nq == 0) return PANN_OK;
+    if (ids) HIP_TRY(hipMemcpyAsync(ids, s->out.ids, nk, hipMemcpyDeviceToHost, st));
+    if (dists) HIP_TRY(hipMemcpyAsync(dists, s->out.dists, nk, hipMemcpyDeviceToHost, st));
+    if (comps) HIP_TRY(hipMemcpyAsync(comps, s->out.comps, nn, hipMemcpyDeviceToHost, st));
+    if (flags) HIP_TRY(hipMemcpyAsync(flags, s->out.flags, nn, hipMemcpyDeviceToHost, st));
+    if (edges) HIP_TRY(hipMemcpyAsync(edges, s->out.edges, nn, hipMemcpyDeviceToHost, st));
+    if (hops) HIP_TRY(hipMemcpyAsync(hops, s->out.hops, nn, hipMemcpyDeviceToHost, st));
+    if (!stream) HIP_TRY(hipStreamSynchronize(nullptr));
+    return PANN_OK;
+}
+
+void pann_searcher_free(pann_searcher* s) {
+    if (!s) return;
+    freeOutputs(s->out);
+    hipFree(s->gfilt);
+    delete s;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// pann_internal.h -- structures shared by the HIP kernels and the host library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pann {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // padding of adjacency rows / missing neighbour
+constexpr int kVecBytes = 128;            // device row pitch of base and query vectors
+constexpr int kMaxDim = 128;              // widest vector the kernels accept
+constexpr int kWave = 64;
+constexpr int kTagSlots = 1024;           // per-wave byte table used to detect slot collisions
+constexpr int kFlagOverflow = 1;          // cuda/busqueda.cuh:14
+constexpr int kFlagShort = 2;             // cuda/busqueda.cuh:15
+constexpr int kFlagConfig = 4;            // query parameters outside what the launch was sized for
+
+// Per-query parameters of the batched form (one record per query).
+struct QueryParams {
+    uint32_t qid;         // id used by the "a == query index" exclusion
+    int32_t Ls;
+    int32_t k;
+    int32_t pad_;
+    int64_t visit_limit;  // <= 0: unlimited (n)
+    double cut;
+};
+static_assert(sizeof(QueryParams) == 32, "QueryParams layout is part of the C ABI");
+
+// Byte offsets of the per-wave LDS regions of the flat search.
+struct FlatLds {
+    uint32_t fkey[2];  // frontier keys (d << 32 | id), two buffers of LsMax
+    uint32_t fvis[2];  // frontier visited flags (u8), two buffers
+    uint32_t occ;      // u8 per merged position
+    uint32_t kkey;     // entering candidate keys (64*maxch)
+    uint32_t kvis;     // u8
+    uint32_t pend;     // pending ids after the seen filter (64*maxch u32)
+    uint32_t tag;      // kTagSlots u8
+    uint32_t vkey;     // visited hash keys (capV u32)
+    uint32_t vcnt;     // visited hash counts (capV u8)
+    uint32_t filt;     // seen filter (1 << fbitsLds u32), when held in LDS
+    uint32_t total;    // bytes per wave
+};
+
+__host__ __device__ inline uint32_t alignUp(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
+
+__host__ __device__ inline FlatLds flatLdsLayout(int LsMax, int maxch, uint32_t capV,
+                                                 int fbitsLds) {
+    FlatLds L;
+    uint32_t o = 0;
+    const uint32_t cmax = 64u * (uint32_t)maxch;
+    L.fkey[0] = o; o += 8u * LsMax;
+    L.fkey[1] = o; o += 8u * LsMax;
+    L.kkey = o;    o += 8u * cmax;
+    L.pend = o;    o += 4u * cmax;
+    L.vkey = o;    o += 4u * capV;
+    L.filt = o;    o += (fbitsLds > 0) ? (4u << fbitsLds) : 0u;
+    L.fvis[0] = o; o += alignUp(LsMax, 8);
+    L.fvis[1] = o; o += alignUp(LsMax, 8);
+    L.occ = o;     o += alignUp(LsMax, 8);
+    L.kvis = o;    o += alignUp(cmax, 8);
+    L.tag = o;     o += kTagSlots;
+    L.vcnt = o;    o += alignUp(capV, 8);
+    L.total = alignUp(o, 16);
+    return L;
+}
+
+// bits of ParlayANN's seen filter: max(10, ceil(log2(Ls*Ls)) - 2)   cuda/busqueda.cu:536-538
+__host__ __device__ inline int ceilLog2(uint64_t x) {
+    int b = 0;
+    while (((uint64_t)1 << b) < x) b++;
+    return b;
+}
+__host__ __device__ inline int flatFilterBits(int Ls) {
+    int b = ceilLog2((uint64_t)Ls * (uint64_t)Ls) - 2;
+    return b < 10 ? 10 : b;
+}
+// visited table capacity: nextpow2(8*Ls + 128)                       cuda/busqueda.cu:540
+__host__ __device__ inline uint32_t flatVisitedCap(int Ls) {
+    return 1u << ceilLog2((uint64_t)8 * Ls + 128);
+}
+
+struct FlatLaunch {
+    const int8_t* base;        // n rows, kVecBytes pitch, zero padded
+    const uint32_t* adj;       // n x G, kEmpty padded
+    const int8_t* queries;     // nq rows, kVecBytes pitch
+    const uint32_t* qids;      // nullable: query index = row
+    const QueryParams* qp;     // nullable: uniform parameters below
+    uint32_t n, nq, G;
+    int32_t Ls, k;
+    int64_t limit;
+    double cut;
+    // sizing of this launch
+    int32_t LsMax;             // frontier capacity in LDS
+    int32_t fbitsLds;          // filters up to this many bits live in LDS
+    int32_t gbitsMax;          // global filter region bits per wave (0: none)
+    uint32_t capV;             // visited hash capacity in LDS (power of two)
+    uint32_t ldsPerWave;
+    FlatLds lds;
+    uint32_t* gfilt;           // per-wave global filter regions
+    uint32_t* queue;           // persistent mode: {next query, finished waves}; null: static
+    uint32_t totalWaves;
+    // outputs
+    uint32_t* outIds;
+    int32_t* outDists;
+    uint32_t outStride;
+    int32_t* outComps;
+    int32_t* outFlags;
+    int32_t* outEdges;
+    int32_t* outHops;
+};
+
+// Launchers implemented in the .hip translation units.
+hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int blocks,
+                          hipStream_t stream);
+hipError_t beamFlatOccupancy(int maxch, int wavesPerBlock, size_t ldsBytes, int* blocksPerCU);
+
+}  // namespace pann

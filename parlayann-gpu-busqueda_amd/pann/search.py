@@ -1,0 +1,290 @@
+"""Host-side mirror of the reference's search interface over the C ABI.
+
+    Vectors        VectoresGPU + subirVectores / copiarVectoresAsync   (cuda/datos_gpu.h:8-14)
+    FlatGraph      GrafoGPU + subirGrafo                               (cuda/datos_gpu.h:19-28)
+    HnswIndex      HnswGPU + leerHnsw/subirHnsw                        (cuda/hnsw.h:43-62)
+    SearchConfig   ConfigBusqueda                                      (cuda/busqueda.cuh:6-11)
+    Searcher       BuscadorGPU: prepare / launch / free                (cuda/busqueda.cuh:36-62)
+    HnswSearcher   BuscadorHnsw: prepare / launch / free               (cuda/hnsw.h:65-86)
+
+Everything runs through libpann.so; there is no other compute path.  Errors raise
+PannError (the reference exits the process instead).
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import PannError, check, lib
+
+NO_NEIGHBOR = _lib.NO_NEIGHBOR
+FLAG_OVERFLOW = _lib.FLAG_OVERFLOW
+FLAG_SHORT = _lib.FLAG_SHORT
+FLAG_CONFIG = _lib.FLAG_CONFIG
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def device_count():
+    n = C.c_int(0)
+    code = lib().pann_device_count(C.byref(n))
+    return n.value if code == 0 else 0
+
+
+def synchronize(stream=None):
+    check(lib().pann_synchronize(stream))
+
+
+class DeviceBuffer:
+    """Plain device allocation (pann_device_alloc)."""
+
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        check(lib().pann_device_alloc(nbytes, C.byref(p)))
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @staticmethod
+    def from_array(a, stream=None):
+        a = np.ascontiguousarray(a)
+        b = DeviceBuffer(a.nbytes)
+        check(lib().pann_memcpy(b.ptr, _ptr(a), a.nbytes, stream))
+        return b
+
+    def to_array(self, dtype, shape, stream=None):
+        out = np.empty(shape, dtype=dtype)
+        check(lib().pann_memcpy(_ptr(out), self.ptr, out.nbytes, stream))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().pann_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Vectors:
+    """Device rows with a 128-byte pitch (VectoresGPU)."""
+
+    def __init__(self, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.int8)
+        h = C.c_void_p()
+        check(lib().pann_vectors_create(_ptr(rows), rows.shape[0], rows.shape[1], C.byref(h)))
+        self.handle = h.value
+        self.n, self.dim = rows.shape
+
+    @classmethod
+    def load(cls, path):
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        check(lib().pann_vectors_load(str(path).encode(), C.byref(h)))
+        self.handle = h.value
+        n, d = C.c_uint32(), C.c_uint32()
+        check(lib().pann_vectors_info(self.handle, C.byref(n), C.byref(d), None, None))
+        self.n, self.dim = n.value, d.value
+        return self
+
+    def device_rows(self):
+        p = C.c_void_p()
+        check(lib().pann_vectors_info(self.handle, None, None, C.byref(p), None))
+        return p.value
+
+    def copy_async(self, rows, stream=None):
+        rows = np.ascontiguousarray(rows, dtype=np.int8)
+        check(lib().pann_vectors_copy_async(self.handle, _ptr(rows), rows.shape[0], stream))
+
+    def free(self):
+        if self.handle:
+            lib().pann_vectors_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class FlatGraph:
+    """Dense n x max_degree adjacency in HBM (GrafoGPU)."""
+
+    def __init__(self, graph):
+        h = C.c_void_p()
+        deg = np.ascontiguousarray(graph.degrees, dtype=np.uint32)
+        edges = np.ascontiguousarray(graph.edges, dtype=np.uint32)
+        check(lib().pann_graph_create(graph.n, graph.max_degree, _ptr(deg), _ptr(edges), C.byref(h)))
+        self.handle = h.value
+        self.n, self.max_degree = graph.n, max(1, graph.max_degree)
+
+    @classmethod
+    def load(cls, path):
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        check(lib().pann_graph_load(str(path).encode(), C.byref(h)))
+        self.handle = h.value
+        n, g = C.c_uint32(), C.c_uint32()
+        check(lib().pann_graph_info(self.handle, C.byref(n), C.byref(g)))
+        self.n, self.max_degree = n.value, g.value
+        return self
+
+    def free(self):
+        if self.handle:
+            lib().pann_graph_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HnswIndex:
+    """HNSW layers in HBM with vectors in node order (HnswGPU)."""
+
+    def __init__(self, h, base):
+        out = C.c_void_p()
+        arrs = [np.ascontiguousarray(h.level, dtype=np.uint32),
+                np.ascontiguousarray(h.node_point, dtype=np.uint32),
+                np.ascontiguousarray(h.first_list, dtype=np.uint64),
+                np.ascontiguousarray(h.list_offsets, dtype=np.uint64),
+                np.ascontiguousarray(h.list_sizes, dtype=np.uint32),
+                np.ascontiguousarray(h.list_ids, dtype=np.uint32),
+                np.ascontiguousarray(h.entries, dtype=np.uint32)]
+        check(lib().pann_hnsw_create(h.n, *[_ptr(a) for a in arrs[:6]], _ptr(arrs[6]),
+                                     len(h.entries), base.handle, C.byref(out)))
+        self._keep = arrs
+        self.handle = out.value
+        self.n = h.n
+
+    @classmethod
+    def load(cls, path, base):
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        check(lib().pann_hnsw_load(str(path).encode(), base.handle, C.byref(h)))
+        self.handle = h.value
+        n = C.c_uint32()
+        check(lib().pann_hnsw_info(self.handle, C.byref(n), None, None, None, None))
+        self.n = n.value
+        return self
+
+    def free(self):
+        if self.handle:
+            lib().pann_hnsw_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class SearchConfig:
+    """ConfigBusqueda: Ls, visit limit (None = n), k, cut."""
+    Ls: int = 10
+    k: int = 10
+    visit_limit: int = None
+    cut: float = 1.35
+
+    def c(self):
+        return _lib.SearchConfig(self.Ls, self.k, int(self.visit_limit or 0), float(self.cut))
+
+
+@dataclass
+class Results:
+    ids: np.ndarray
+    dists: np.ndarray
+    comps: np.ndarray
+    flags: np.ndarray
+    edges: np.ndarray
+    hops: np.ndarray
+
+
+class _SearcherBase:
+    handle = None
+
+    def outputs(self):
+        o = _lib.SearchOutputs()
+        check(lib().pann_searcher_outputs(self.handle, C.byref(o)))
+        return o
+
+    def download(self, stream=None):
+        """Copies the last launch's outputs to host arrays (blocking when stream is None)."""
+        nq, k = self.nq, max(1, self.cfg.k)
+        r = Results(np.empty((nq, k), np.uint32), np.empty((nq, k), np.int32),
+                    np.empty(nq, np.int32), np.empty(nq, np.int32), np.empty(nq, np.int32),
+                    np.empty(nq, np.int32))
+        check(lib().pann_searcher_download(self.handle, _ptr(r.ids), _ptr(r.dists), _ptr(r.comps),
+                                           _ptr(r.flags), _ptr(r.edges), _ptr(r.hops), stream))
+        if stream is not None:
+            synchronize(stream)
+        r.ids, r.dists = r.ids[:, :self.cfg.k], r.dists[:, :self.cfg.k]
+        return r
+
+    def free(self):
+        if self.handle:
+            lib().pann_searcher_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Searcher(_SearcherBase):
+    """BuscadorGPU: prepare(graph, nq, cfg) once, launch(...) many times."""
+
+    def __init__(self, graph, nq, cfg):
+        h = C.c_void_p()
+        cc = cfg.c()
+        check(lib().pann_searcher_prepare(graph.handle, nq, C.byref(cc), C.byref(h)))
+        self.handle = h.value
+        self.nq, self.cfg = nq, cfg
+
+    def launch(self, base, queries, graph, qids=None, stream=None):
+        """Asynchronous on `stream`; qids is an optional DeviceBuffer of nq u32."""
+        check(lib().pann_searcher_launch(self.handle, base.handle, queries.handle, graph.handle,
+                                         qids.ptr if qids is not None else None, stream))
+
+
+class HnswSearcher(_SearcherBase):
+    """BuscadorHnsw: prepare(index, nq, cfg) once, launch(...) many times."""
+
+    def __init__(self, index, nq, cfg):
+        h = C.c_void_p()
+        cc = cfg.c()
+        check(lib().pann_hnsw_searcher_prepare(index.handle, nq, C.byref(cc), C.byref(h)))
+        self.handle = h.value
+        self.nq, self.cfg = nq, cfg
+
+    def launch(self, queries, index, stream=None):
+        check(lib().pann_hnsw_searcher_launch(self.handle, queries.handle, index.handle, stream))
+
+
+def search_flat(base, graph, queries, cfg, qids=None):
+    """One blocking search: returns Results with ids/dists of shape [nq, k]."""
+    s = Searcher(graph, queries.n, cfg)
+    try:
+        qb = DeviceBuffer.from_array(np.asarray(qids, dtype=np.uint32)) if qids is not None else None
+        s.launch(base, queries, graph, qb)
+        synchronize()
+        return s.download()
+    finally:
+        s.free()
+
+
+__all__ = ["Vectors", "FlatGraph", "HnswIndex", "SearchConfig", "Searcher", "HnswSearcher",
+           "DeviceBuffer", "Results", "search_flat", "device_count", "synchronize", "PannError",
+           "NO_NEIGHBOR", "FLAG_OVERFLOW", "FLAG_SHORT", "FLAG_CONFIG"]
