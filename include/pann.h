@@ -158,6 +158,14 @@ int pann_searcher_download(const pann_searcher* s, uint32_t* ids, int32_t* dists
                            int32_t* flags, int32_t* edges, int32_t* hops, void* stream);
 void pann_searcher_free(pann_searcher* s); /* liberar */
 
+/* ---- exact ground truth ---- */
+/* Exact top-k (k <= 128) of the first nq queries over all base rows, ordered by
+ * (distance, id), written to device arrays ids/dists (nq x k).  int8 -> int32 MFMA
+ * distances with fused selection.  The reference reads precomputed ground truth instead
+ * (leerGroundTruth, cuda/lectores.cpp:146-170).  Blocking. */
+int pann_bruteforce_knn(const pann_vectors* base, const pann_vectors* queries, uint32_t nq,
+                        uint32_t k, uint32_t* ids_dev, int32_t* dists_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -586,6 +586,29 @@ int pann_searcher_download(const pann_searcher* s, uint32_t* ids, int32_t* dists
     return PANN_OK;
 }
 
+int pann_bruteforce_knn(const pann_vectors* base, const pann_vectors* queries, uint32_t nq,
+                        uint32_t k, uint32_t* ids_dev, int32_t* dists_dev, void* stream) {
+    if (!base || !queries || (nq && (!ids_dev || !dists_dev)))
+        return setError(PANN_ERR_ARG, "NULL argument");
+    if (base->dim != queries->dim) return setError(PANN_ERR_ARG, "dim mismatch");
+    if (nq > queries->n) return setError(PANN_ERR_ARG, "nq %u > %u query rows", nq, queries->n);
+    if (k == 0 || k > 128) return setError(PANN_ERR_CAPACITY, "k=%u outside 1..128", k);
+    if (nq == 0) return PANN_OK;
+    const hipStream_t st = (hipStream_t)stream;
+    int32_t *bn = nullptr, *qn = nullptr;
+    hipError_t e = hipMalloc(&bn, 4ull * std::max<uint32_t>(1, base->n));
+    if (e == hipSuccess) e = hipMalloc(&qn, 4ull * nq);
+    if (e == hipSuccess) e = launchRowNorms(base->dev, base->n, bn, st);
+    if (e == hipSuccess) e = launchRowNorms(queries->dev, nq, qn, st);
+    if (e == hipSuccess)
+        e = launchBruteForceKnn(base->dev, bn, base->n, queries->dev, qn, nq, k, ids_dev, dists_dev, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(bn);
+    hipFree(qn);
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "brute-force k-NN: %s", hipGetErrorString(e));
+    return PANN_OK;
+}
+
 void pann_searcher_free(pann_searcher* s) {
     if (!s) return;
     freeOutputs(s->out);

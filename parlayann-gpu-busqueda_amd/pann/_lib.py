@@ -88,6 +88,7 @@ PANN_SIGNATURES = {
     "pann_searcher_outputs": (C.c_int, [_P, C.POINTER(SearchOutputs)]),
     "pann_searcher_download": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "pann_searcher_free": (None, [_P]),
+    "pann_bruteforce_knn": (C.c_int, [_P, _P, _u32, _u32, _P, _P, _P]),
 }
 
 TOOLS_SIGNATURES = {

@@ -285,6 +285,18 @@ def search_flat(base, graph, queries, cfg, qids=None):
         s.free()
 
 
-__all__ = ["Vectors", "FlatGraph", "HnswIndex", "SearchConfig", "Searcher", "HnswSearcher",
+def bruteforce_knn(base, queries, k, nq=None, stream=None):
+    """Exact top-k on the GPU (MFMA distances, fused selection): ids u32, dists i32 [nq, k]."""
+    nq = queries.n if nq is None else nq
+    ids = DeviceBuffer(max(1, nq * k * 4))
+    d = DeviceBuffer(max(1, nq * k * 4))
+    try:
+        check(lib().pann_bruteforce_knn(base.handle, queries.handle, nq, k, ids.ptr, d.ptr, stream))
+        return ids.to_array(np.uint32, (nq, k)), d.to_array(np.int32, (nq, k))
+    finally:
+        ids.free(), d.free()
+
+
+__all__ = ["bruteforce_knn", "Vectors", "FlatGraph", "HnswIndex", "SearchConfig", "Searcher", "HnswSearcher",
            "DeviceBuffer", "Results", "search_flat", "device_count", "synchronize", "PannError",
            "NO_NEIGHBOR", "FLAG_OVERFLOW", "FLAG_SHORT", "FLAG_CONFIG"]
