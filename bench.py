@@ -94,22 +94,37 @@ def time_launches(searcher, b, q, g, steps, warmup):
 
 
 def cpu_baseline(base, graph, queries, Ls, k, seconds):
-    """Oracle on all host cores over a bounded sample sized to about `seconds`."""
+    """Oracle on all host cores: one warm-up pass over the query set, then repeated
+    passes for about `seconds`, best pass reported (the reference's best-of-N protocol)."""
     from oracle import oracle as O
     cores = host_cores()
     nq = len(queries)
-    probe = min(nq, 2000)
-    t = time.perf_counter()
-    O.beam_flat(base, graph, queries[:probe], O.make_params(probe, Ls, k), nthreads=cores)
-    dt = max(1e-6, time.perf_counter() - t)
-    sample = int(min(nq, max(probe, probe * seconds / dt)))
-    params = O.make_params(sample, Ls, k)
-    t = time.perf_counter()
-    O.beam_flat(base, graph, queries[:sample], params, nthreads=cores)
-    dt = time.perf_counter() - t
-    return {"value": round(sample / dt, 1), "unit": "QPS", "cores": cores, "kind": "port",
-            "sample": f"first {sample} of {nq} queries at Ls={Ls}, k={k}, {dt:.1f}s on {cores} "
-                      f"threads (oracle: C++ restatement of ParlayANN beam_search)"}
+    params = O.make_params(nq, Ls, k)
+    O.beam_flat(base, graph, queries, params, nthreads=cores)
+    best, passes, t_all = 0.0, 0, time.perf_counter()
+    while passes < 5 or time.perf_counter() - t_all < seconds:
+        t = time.perf_counter()
+        O.beam_flat(base, graph, queries, params, nthreads=cores)
+        best = max(best, nq / (time.perf_counter() - t))
+        passes += 1
+        if passes >= 200:
+            break
+    spent = time.perf_counter() - t_all
+    return {"value": round(best, 1), "unit": "QPS", "cores": cores, "kind": "port",
+            "sample": f"all {nq} queries at Ls={Ls}, k={k}: warm-up + best of {passes} passes "
+                      f"({spent:.1f}s) on {cores} threads; oracle = C++ restatement of "
+                      f"ParlayANN beam_search"}
+
+
+def calibrate_copy(nbytes, reps):
+    """Runs the plain copy kernel `reps` times over `nbytes` (byte-counter calibration)."""
+    from pann import _lib, search
+    L = _lib.lib()
+    src, dst = search.DeviceBuffer(nbytes), search.DeviceBuffer(nbytes)
+    for _ in range(reps):
+        _lib.check(L.pann_copy_kernel(dst.ptr, src.ptr, nbytes, None))
+    search.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": nbytes, "reps": reps}))
 
 
 def main():
@@ -129,7 +144,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run the copy kernel over this many bytes (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy, 3)
+        return
 
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the search shards by "

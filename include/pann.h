@@ -96,6 +96,9 @@ int pann_event_create(void** out);
 int pann_event_record(void* ev, void* stream);
 int pann_event_elapsed_ms(void* start, void* end, float* ms); /* waits for `end` */
 void pann_event_destroy(void* ev);
+/* Plain streaming copy kernel (16-byte vector loads/stores, bytes % 16 == 0): a launch of
+ * known traffic used to scale hardware byte counters. */
+int pann_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
 
 /* ---- vectors (VectoresGPU) ---- */
 /* subirVectores (cuda/datos_gpu.cu:20-40): host rows of `dim` int8 -> device rows with a

@@ -200,7 +200,23 @@ __global__ __launch_bounds__(64) void bfKnnKernel(const int8_t* base, const int3
     }
 }
 
+__global__ __launch_bounds__(256) void copyKernel(const int4* __restrict__ src,
+                                                  int4* __restrict__ dst, size_t n16) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    const size_t n16 = bytes / 16;
+    if (n16 == 0) return hipSuccess;
+    const size_t want = (n16 + 255) / 256, blocks = want < 256 * 64 ? want : 256 * 64;
+    hipLaunchKernelGGL(copyKernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                       reinterpret_cast<const int4*>(src), reinterpret_cast<int4*>(dst), n16);
+    return hipGetLastError();
+}
 
 hipError_t launchRowNorms(const int8_t* rows, uint32_t n, int32_t* out, hipStream_t s) {
     if (n == 0) return hipSuccess;

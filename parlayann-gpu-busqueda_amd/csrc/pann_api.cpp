@@ -219,6 +219,12 @@ void pann_event_destroy(void* ev) {
     if (ev) hipEventDestroy((hipEvent_t)ev);
 }
 
+int pann_copy_kernel(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!dst || !src || bytes % 16) return setError(PANN_ERR_ARG, "copy needs 16-byte multiples");
+    HIP_TRY(launchCopy(dst, src, bytes, (hipStream_t)stream));
+    return PANN_OK;
+}
+
 // ---------------------------------------------------------------- vectors
 int pann_vectors_create(const int8_t* host_rows, uint32_t n, uint32_t dim, pann_vectors** out) {
     if (!out) return setError(PANN_ERR_ARG, "out is NULL");

@@ -67,6 +67,7 @@ PANN_SIGNATURES = {
     "pann_event_record": (C.c_int, [_P, _P]),
     "pann_event_elapsed_ms": (C.c_int, [_P, _P, C.POINTER(C.c_float)]),
     "pann_event_destroy": (None, [_P]),
+    "pann_copy_kernel": (C.c_int, [_P, _P, C.c_size_t, _P]),
     "pann_vectors_create": (C.c_int, [_P, _u32, _u32, _PP]),
     "pann_vectors_load": (C.c_int, [C.c_char_p, _PP]),
     "pann_vectors_copy_async": (C.c_int, [_P, _P, _u32, _P]),
