@@ -229,6 +229,8 @@ int oracle_search_hnsw(const int8_t* base, uint32_t dim, const oracle_hnsw* h,
     };
     std::vector<uint32_t> eps(h->entries, h->entries + h->num_entries);
     const int top = (int)h->level[eps[0]];
+    for (uint32_t e : eps)
+        if (e >= n || (int)h->level[e] != top) return 1;  // entry points live on the top layer
     int64_t comps = 0, edgesRead = 0, hops = 0;
     std::vector<IdDist> W;
 

@@ -13,159 +13,15 @@
 //     v_dot4_i32_i8 for q.p and p.p (no norm array), DPP transpose-reduce;
 //   * candidates: in-register bitonic sort of (d << 32 | id) keys;
 //   * merge: every merged position computes its source with a ballot rank (no shifts).
-#include "pann_internal.h"
+#include "device_common.h"
 
 namespace pann {
 namespace {
 
-__device__ __forceinline__ uint64_t hash64_2(uint64_t x) {
-    // parlaylib hash64_2 (cuda/utiles_gpu.cuh:12-17)
-    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
-    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
-    return x ^ (x >> 31);
-}
-
-__device__ __forceinline__ int laneId() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0)); }
-
-__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-}
-
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
-
-__device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
-    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
-    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ uint64_t shflXor64(uint64_t v, int m) {
-    uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
-    uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// DPP moves inside 8-lane groups.
-__device__ __forceinline__ int dppHalfMirror(int x) {  // lane i <- lane 7-i
-    return __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false);
-}
-__device__ __forceinline__ int dppXor2(int x) {  // quad_perm [2,3,0,1]
-    return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);
-}
-__device__ __forceinline__ int dppXor1(int x) {  // quad_perm [1,0,3,2]
-    return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
-}
-
-// Sum of x over the 8-lane group, in every lane of the group.
-__device__ __forceinline__ int groupSum8(int x) {
-    x += dppXor1(x);
-    x += dppXor2(x);
-    x += dppHalfMirror(x);
-    return x;
-}
-
-// v[r] holds lane's partial for round r.  Returns, in lane l, the 8-lane group total of
-// round (l & 7): a 3-step transpose-reduce (7 DPP moves instead of 24).
-__device__ __forceinline__ int transposeReduce8(const int (&v)[8], int lane) {
-    const bool b2 = lane & 4, b1 = lane & 2, b0 = lane & 1;
-    int w[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        int send = b2 ? v[j] : v[j + 4];
-        int keep = b2 ? v[j + 4] : v[j];
-        w[j] = keep + dppHalfMirror(send);
-    }
-    int x[2];
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-        int send = b1 ? w[j] : w[j + 2];
-        int keep = b1 ? w[j + 2] : w[j];
-        x[j] = keep + dppXor2(send);
-    }
-    int send = b0 ? x[0] : x[1];
-    int keep = b0 ? x[1] : x[0];
-    return keep + dppXor1(send);
-}
-
-__device__ __forceinline__ int sdot4(int a, int b, int c) {
-    return __builtin_amdgcn_sdot4(a, b, c, false);
-}
-
-// p.p - 2 q.p over one 16-byte slice (exact int32).
-__device__ __forceinline__ int slicePartial(const int4& v, const int4& q) {
-    int pp = sdot4(v.x, v.x, 0);
-    pp = sdot4(v.y, v.y, pp);
-    pp = sdot4(v.z, v.z, pp);
-    pp = sdot4(v.w, v.w, pp);
-    int qp = sdot4(v.x, q.x, 0);
-    qp = sdot4(v.y, q.y, qp);
-    qp = sdot4(v.z, q.z, qp);
-    qp = sdot4(v.w, q.w, qp);
-    return pp - 2 * qp;
-}
-
-// Ascending bitonic sort of 64*C keys, element e = c*64 + lane in key[c].
-template <int C>
-__device__ __forceinline__ void bitonicSort(uint64_t (&key)[C], int lane) {
-    constexpr int N = 64 * C;
-#pragma unroll
-    for (int k = 2; k <= N; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 64) {
-#pragma unroll
-                for (int c = 0; c < C; c++) {
-                    const int c2 = c ^ (j >> 6);
-                    if (c < c2) {
-                        const int e = c * 64 + lane;
-                        const bool asc = (e & k) == 0;
-                        uint64_t a = key[c], b = key[c2];
-                        bool sw = asc ? (a > b) : (a < b);
-                        key[c] = sw ? b : a;
-                        key[c2] = sw ? a : b;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int c = 0; c < C; c++) {
-                    const int e = c * 64 + lane;
-                    const bool asc = (e & k) == 0;
-                    const bool lower = (lane & j) == 0;
-                    uint64_t p = shflXor64(key[c], j);
-                    uint64_t mn = key[c] < p ? key[c] : p;
-                    uint64_t mx = key[c] < p ? p : key[c];
-                    key[c] = (lower == asc) ? mn : mx;
-                }
-            }
-        }
-    }
-}
+using namespace dev;
 
 __device__ __forceinline__ uint32_t visitedSlot(uint32_t id, uint32_t logCap) {
     return (id * 2654435761u) >> (32 - logCap);
-}
-
-// Distance from the query slice to one base row, summed over the 8-lane group.
-__device__ __forceinline__ int groupDistance(const int8_t* base, uint32_t id, int sub,
-                                             const int4& qv, int qq) {
-    const int4 v = *reinterpret_cast<const int4*>(base + (size_t)id * kVecBytes + sub * 16);
-    return qq + groupSum8(slicePartial(v, qv));
-}
-
-// Seen-filter accesses.  An LDS filter uses plain ds ops; a per-wave HBM filter is read
-// and written at agent scope so the wave's own earlier writes are visible to it (the
-// vector L1 is not coherent with its stores).
-template <bool G>
-__device__ __forceinline__ uint32_t filtLoad(const uint32_t* f, uint32_t h) {
-    if constexpr (G) return __hip_atomic_load(f + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else return f[h];
-}
-template <bool G>
-__device__ __forceinline__ void filtStore(uint32_t* f, uint32_t h, uint32_t v) {
-    if constexpr (G) __hip_atomic_store(f + h, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else f[h] = v;
 }
 
 struct QueryCfg {
@@ -205,22 +61,12 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
 
     // ---- query slice: lane holds bytes [16*sub, 16*sub+16) ----
     const int4 qv = *reinterpret_cast<const int4*>(L.queries + (size_t)q * kVecBytes + sub * 16);
-    int qq = sdot4(qv.x, qv.x, 0);
-    qq = sdot4(qv.y, qv.y, qq);
-    qq = sdot4(qv.z, qv.z, qq);
-    qq = sdot4(qv.w, qv.w, qq);
-    qq = groupSum8(qq);
+    const int qq = groupSum8(sliceNorm(qv));
 
     // ---- fresh tables for this query (the CPU creates them per query) ----
     {
-        uint4 e4 = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
-        if constexpr (FGLOBAL) {
-            for (uint32_t i = lane; i < (1u << fbits); i += 64) filtStore<true>(filt, i, kEmpty);
-            __builtin_amdgcn_s_waitcnt(0);
-        } else {
-            uint4* f4 = reinterpret_cast<uint4*>(filt);
-            for (uint32_t i = lane; i < (1u << fbits) / 4; i += 64) f4[i] = e4;
-        }
+        const uint4 e4 = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+        filtClear<FGLOBAL>(filt, fbits, lane);
         uint4* v4 = reinterpret_cast<uint4*>(vkey);
         for (uint32_t i = lane; i < capV / 4; i += 64) v4[i] = e4;
         for (int i = lane; i < Ls; i += 64) occ[i] = 0;
@@ -291,35 +137,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
             const uint32_t ai = a[ch];
             edges += __popcll(ballot(ai != kEmpty));
             const bool part = (ai != kEmpty) && (ai != c.qid);
-            const uint32_t h = (uint32_t)hash64_2(ai) & fmask;
-            const uint32_t ts = h & (kTagSlots - 1);
-            if (part) tag[ts] = (unsigned char)lane;
-            __builtin_amdgcn_wave_barrier();
-            uint32_t prev = kEmpty;
-            bool coll = false;
-            if (part) {
-                coll = tag[ts] != (unsigned char)lane;
-                prev = filtLoad<FGLOBAL>(filt, h);
-            }
-            uint64_t cm = ballot(coll);
-            bool last = true;
-            while (cm) {
-                const int Lr = __ffsll((unsigned long long)cm) - 1;
-                const uint32_t hL = (uint32_t)readlane((int)h, Lr);
-                const uint64_t M = ballot(part && h == hL);
-                const bool inM = (M >> lane) & 1ull;
-                const uint64_t below = M & lanesBelow;
-                const uint64_t above = M & ~(lanesBelow | (1ull << lane));
-                const int src = (inM && below) ? (63 - __clzll((long long)below)) : lane;
-                const uint32_t x = (uint32_t)__shfl((int)ai, src, 64);
-                if (inM && below) prev = x;
-                if (inM && above) last = false;
-                cm &= ~M;
-            }
-            const bool seen = part && (prev == ai);
-            if (part && last) filtStore<FGLOBAL>(filt, h, ai);
-            if constexpr (FGLOBAL) __builtin_amdgcn_s_waitcnt(0);
-            const bool keep = part && !seen;
+            const bool keep = seenFilterChunk<FGLOBAL>(filt, tag, fmask, ai, part, lane, lanesBelow);
             const uint64_t km = ballot(keep);
             if (keep) pend[nP + mbcnt(km)] = ai;
             nP += (uint32_t)__popcll(km);

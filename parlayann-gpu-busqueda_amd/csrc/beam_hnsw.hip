@@ -1,0 +1,424 @@
+// beam_hnsw.hip -- ParlayANN HNSW::search for gfx950: one 64-lane wavefront per query,
+// upper-layer greedy descent (ef = 1) and the layer-0 beam (ef = Ls) in one kernel.
+//
+// Semantics follow the reference's restatement (cuda/busqueda_hnsw.cu:179-457):
+//   * per layer: fresh lossy visited filter (hash64 marks entry points, hash64_2 the
+//     neighbours; 2 bits above layer 0, 2*ceil(log2 ef)-2 on layer 0), fresh is_inw set;
+//   * cand ordered and unique by distance, W (the workset) ordered by (distance, id);
+//   * loop: stop when cand's nearest is farther than W's farthest or the visit limit is
+//     hit; expand the nearest candidate; neighbours are accepted in row order when
+//     |W| < ef or d < W's farthest and they never entered W before in this layer;
+//   * the next layer starts from W's farthest element; output W[0:k] as base point ids.
+// Layout choices of this file: W and cand live in registers as wave-wide sorted lists
+// (element e = r*64 + lane), so an insertion position is one ballot popcount and an
+// insertion a lane shift; is_inw is an LDS open-addressing hash; the sequential
+// acceptance only visits neighbours that can still pass the threshold, found by ballot.
+#include "device_common.h"
+
+namespace pann {
+namespace {
+
+using namespace dev;
+
+// Sorted u64 keys (d << 32 | id) spread over the wave: element e = r*64 + lane in v[r].
+template <int WR>
+struct WaveList {
+    uint64_t v[WR];
+    int size;
+
+    __device__ void clear() {
+#pragma unroll
+        for (int r = 0; r < WR; r++) v[r] = ~0ull;
+        size = 0;
+    }
+    __device__ uint64_t get(int e) const {  // e uniform
+        uint64_t x = ~0ull;
+#pragma unroll
+        for (int r = 0; r < WR; r++)
+            if (r == (e >> 6)) x = readlane64(v[r], e & 63);
+        return x;
+    }
+    __device__ int countLess(uint64_t key, int lane) const {
+        int c = 0;
+#pragma unroll
+        for (int r = 0; r < WR; r++) c += __popcll(ballot(r * 64 + lane < size && v[r] < key));
+        return c;
+    }
+    __device__ bool hasDist(uint32_t d, int lane) const {
+        uint64_t m = 0;
+#pragma unroll
+        for (int r = 0; r < WR; r++) m |= ballot(r * 64 + lane < size && (uint32_t)(v[r] >> 32) == d);
+        return m != 0;
+    }
+    __device__ void insertAt(int pos, uint64_t key, int lane) {
+#pragma unroll
+        for (int r = WR - 1; r >= 0; r--) {
+            uint64_t prev = shflUp64(v[r], 1);
+            if (lane == 0) prev = (r > 0) ? readlane64(v[r > 0 ? r - 1 : 0], 63) : ~0ull;
+            const int e = r * 64 + lane;
+            v[r] = e > pos ? prev : (e == pos ? key : v[r]);
+        }
+        size++;
+    }
+    __device__ void popFront(int lane) {
+#pragma unroll
+        for (int r = 0; r < WR; r++) {
+            uint64_t next = shflDown64(v[r], 1);
+            if (lane == 63) next = (r + 1 < WR) ? readlane64(v[r + 1 < WR ? r + 1 : r], 0) : ~0ull;
+            v[r] = next;
+        }
+        size--;
+    }
+};
+
+__device__ __forceinline__ uint32_t inwSlot(uint32_t id, uint32_t logCap) {
+    return (id * 2654435761u) >> (32 - logCap);
+}
+
+struct LayerState {
+    int comps, edges, hops, flags;
+    uint32_t inwCount;
+};
+
+// Expands node u on `layer`: row gather, seen filter, distances, in-order acceptance.
+template <int WR, int MAXCH, bool G>
+__device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* filt, uint32_t fmask,
+                           uint32_t u, int layer, int ef, const int4& qv, int qq,
+                           WaveList<WR>& W, WaveList<WR>& cand, LayerState& st, uint32_t capInw,
+                           uint32_t logCap, int lane, uint64_t lanesBelow) {
+    const int grp = lane >> 3, sub = lane & 7;
+    uint32_t* pend = reinterpret_cast<uint32_t*>(lds + L.lds.pend);
+    uint32_t* inw = reinterpret_cast<uint32_t*>(lds + L.lds.inw);
+    unsigned char* tag = lds + L.lds.tag;
+    const uint32_t width = layer > 0 ? L.GA : L.G0;
+    const uint32_t* row = layer > 0 ? L.adjUp + ((size_t)L.upStart[u] + layer - 1) * L.GA
+                                    : L.adj0 + (size_t)u * L.G0;
+    const uint32_t nch = (width + 63) / 64;
+    uint32_t a[MAXCH];
+#pragma unroll
+    for (int ch = 0; ch < MAXCH; ch++) {
+        const uint32_t j = ch * 64 + lane;
+        a[ch] = (ch < (int)nch && j < width) ? row[j] : kEmpty;
+    }
+    uint32_t nP = 0;
+#pragma unroll
+    for (int ch = 0; ch < MAXCH; ch++) {
+        if (ch >= (int)nch) break;
+        const bool part = a[ch] != kEmpty;
+        st.edges += __popcll(ballot(part));
+        const bool keep = seenFilterChunk<G>(filt, tag, fmask, a[ch], part, lane, lanesBelow);
+        const uint64_t km = ballot(keep);
+        if (keep) pend[nP + mbcnt(km)] = a[ch];
+        nP += (uint32_t)__popcll(km);
+        __builtin_amdgcn_wave_barrier();
+    }
+    st.comps += (int)nP;
+
+    // distances, then move candidate j to lane j of block j/64
+    int dist[MAXCH];
+    uint32_t id[MAXCH];
+    bool valid[MAXCH], memb[MAXCH], acc[MAXCH];
+#pragma unroll
+    for (int b = 0; b < MAXCH; b++) {
+        valid[b] = false;
+        memb[b] = false;
+        acc[b] = false;
+        dist[b] = 0;
+        id[b] = kEmpty;
+        if ((uint32_t)b * 64 >= nP) continue;
+        const uint32_t nb = min(64u, nP - (uint32_t)b * 64);
+        const uint32_t rounds = (nb + 7) / 8;
+        int4 v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            v[r] = make_int4(0, 0, 0, 0);
+            const uint32_t i = (uint32_t)r * 8 + grp;
+            if ((uint32_t)r < rounds && i < nb)
+                v[r] = *reinterpret_cast<const int4*>(L.vec + (size_t)pend[b * 64 + i] * kVecBytes + sub * 16);
+        }
+        int part8[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) part8[r] = slicePartial(v[r], qv);
+        const int tot = transposeReduce8(part8, lane);
+        // lane l holds candidate (l & 7) * 8 + (l >> 3); the map is its own inverse
+        const int mine = __shfl(tot, (lane & 7) * 8 + (lane >> 3), 64);
+        if ((uint32_t)lane < nb) {
+            valid[b] = true;
+            dist[b] = qq + mine;
+            id[b] = pend[b * 64 + lane];
+            // membership before this row's acceptances
+            uint32_t h = inwSlot(id[b], logCap);
+            while (true) {
+                const uint32_t x = inw[h];
+                if (x == id[b]) { memb[b] = true; break; }
+                if (x == kEmpty) break;
+                h = (h + 1) & (capInw - 1);
+            }
+        }
+    }
+
+    // in-order acceptance; only neighbours that can still pass are visited
+    int pos = -1;
+    while (true) {
+        const bool full = W.size >= ef;
+        const uint32_t wl = (uint32_t)(W.get(W.size - 1) >> 32);
+        int j = -1;
+#pragma unroll
+        for (int b = 0; b < MAXCH; b++) {
+            if (j >= 0) break;
+            const int e = b * 64 + lane;
+            const uint64_t m = ballot(valid[b] && e > pos && !memb[b] && (!full || (uint32_t)dist[b] < wl));
+            if (m) j = b * 64 + __ffsll((unsigned long long)m) - 1;
+        }
+        if (j < 0) break;
+        int dj = 0;
+        uint32_t idj = 0;
+#pragma unroll
+        for (int b = 0; b < MAXCH; b++)
+            if (b == (j >> 6)) {
+                dj = readlane(dist[b], j & 63);
+                idj = (uint32_t)readlane((int)id[b], j & 63);
+            }
+        const uint64_t key = ((uint64_t)(uint32_t)dj << 32) | idj;
+        W.insertAt(W.countLess(key, lane), key, lane);
+        if (!cand.hasDist((uint32_t)dj, lane))
+            cand.insertAt(cand.countLess((uint64_t)(uint32_t)dj << 32, lane), key, lane);
+        if (W.size > ef) W.size--;
+        if (cand.size > ef) cand.size--;
+#pragma unroll
+        for (int b = 0; b < MAXCH; b++) {
+            memb[b] = memb[b] || (valid[b] && id[b] == idj);
+            acc[b] = acc[b] || (b * 64 + lane == j);
+        }
+        pos = j;
+    }
+
+    // record the accepted ids in is_inw
+    uint32_t added = 0;
+#pragma unroll
+    for (int b = 0; b < MAXCH; b++) {
+        added += (uint32_t)__popcll(ballot(acc[b]));
+        if (acc[b]) {
+            uint32_t h = inwSlot(id[b], logCap);
+            while (true) {
+                const uint32_t old = atomicCAS(&inw[h], kEmpty, id[b]);
+                if (old == kEmpty || old == id[b]) break;
+                h = (h + 1) & (capInw - 1);
+            }
+        }
+    }
+    st.inwCount += added;
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int WR, int MAXCH, bool FGLOBAL>
+__device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt, uint32_t q,
+                        int Ls, int k, int64_t visit) {
+    const int lane = laneId();
+    const int sub = lane & 7;
+    const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t* inw = reinterpret_cast<uint32_t*>(lds + L.lds.inw);
+    uint32_t* ldsFilt = reinterpret_cast<uint32_t*>(lds + L.lds.filt);
+    const int4 qv = *reinterpret_cast<const int4*>(L.queries + (size_t)q * kVecBytes + sub * 16);
+    const int qq = groupSum8(sliceNorm(qv));
+    const int64_t lim0 = (visit > 0 && visit < (int64_t)L.n) ? visit : (int64_t)L.n;
+    const int bits0 = hnswFilterBits(Ls);
+    uint32_t capInw = hnswInwCap(Ls);
+    if (capInw > L.capInw) capInw = L.capInw;
+    const uint32_t logCap = (uint32_t)ceilLog2(capInw);
+
+    uint32_t eps[8];
+    int numEps = L.numEntries;
+#pragma unroll
+    for (int e = 0; e < 8; e++) eps[e] = L.entries[e];
+    LayerState st{0, 0, 0, 0, 0};
+    WaveList<WR> W, cand;
+    W.clear();
+    cand.clear();
+
+    for (int layer = L.top; layer >= 0; layer--) {
+        const int ef = layer > 0 ? 1 : Ls;
+        const int bits = layer > 0 ? 2 : bits0;
+        const uint32_t fmask = (1u << bits) - 1;
+        const int64_t limit = layer > 0 ? INT64_MAX : lim0;
+        const bool globalFilter = FGLOBAL && layer == 0;
+        uint32_t* filt = globalFilter ? gfilt : ldsFilt;
+        if (globalFilter) filtClear<true>(filt, bits, lane);
+        else filtClear<false>(filt, bits, lane);
+        for (uint32_t i = lane; i < capInw; i += 64) inw[i] = kEmpty;
+        st.inwCount = 0;
+        W.clear();
+        cand.clear();
+        __builtin_amdgcn_wave_barrier();
+
+        for (int e = 0; e < numEps; e++) {
+            const uint32_t pe = eps[e];
+            if (lane == 0) {
+                const uint32_t slot = (uint32_t)hash64(pe) & fmask;
+                if (globalFilter) filtStore<true>(filt, slot, pe);
+                else filtStore<false>(filt, slot, pe);
+                uint32_t h = inwSlot(pe, logCap);
+                while (inw[h] != kEmpty && inw[h] != pe) h = (h + 1) & (capInw - 1);
+                if (inw[h] == kEmpty) {
+                    inw[h] = pe;
+                    st.inwCount++;
+                }
+            }
+            if (globalFilter) __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+            const int d = readlane(groupDistance(L.vec, pe, sub, qv, qq), 0);
+            const uint64_t key = ((uint64_t)(uint32_t)d << 32) | pe;
+            if (!cand.hasDist((uint32_t)d, lane))
+                cand.insertAt(cand.countLess((uint64_t)(uint32_t)d << 32, lane), key, lane);
+            W.insertAt(W.countLess(key, lane), key, lane);
+            st.comps++;
+        }
+        st.inwCount = (uint32_t)readlane((int)st.inwCount, 0);
+
+        int64_t cnt = 0;
+        while (cand.size > 0) {
+            const uint64_t c0 = cand.get(0);
+            const uint64_t wl = W.get(W.size - 1);
+            if ((uint32_t)(c0 >> 32) > (uint32_t)(wl >> 32)) break;
+            if (++cnt > limit) break;
+            cand.popFront(lane);
+            st.hops++;
+            if (globalFilter)
+                expandNode<WR, MAXCH, true>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq, W,
+                                            cand, st, capInw, logCap, lane, lanesBelow);
+            else
+                expandNode<WR, MAXCH, false>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq,
+                                             W, cand, st, capInw, logCap, lane, lanesBelow);
+            if (st.inwCount + 64u * MAXCH >= capInw) {
+                st.flags |= kFlagOverflow;
+                break;
+            }
+        }
+        if (st.flags & kFlagOverflow) break;
+        if (layer > 0) {
+            eps[0] = (uint32_t)W.get(W.size - 1);
+            numEps = 1;
+        }
+    }
+
+    if (W.size < k) st.flags |= kFlagShort;
+    uint32_t* oi = L.outIds + (size_t)q * L.outStride;
+    int32_t* od = L.outDists + (size_t)q * L.outStride;
+#pragma unroll
+    for (int r = 0; r < WR; r++) {
+        const int e = r * 64 + lane;
+        if (e < k) {
+            if (e < W.size) {
+                oi[e] = L.nodePoint[(uint32_t)W.v[r]];
+                od[e] = (int32_t)(W.v[r] >> 32);
+            } else {
+                oi[e] = kEmpty;
+                od[e] = 0x7FFFFFFF;
+            }
+        }
+    }
+    for (int e = WR * 64 + lane; e < k; e += 64) {
+        oi[e] = kEmpty;
+        od[e] = 0x7FFFFFFF;
+    }
+    if (lane == 0) {
+        if (L.outComps) L.outComps[q] = st.comps;
+        if (L.outFlags) L.outFlags[q] = st.flags;
+        if (L.outEdges) L.outEdges[q] = st.edges;
+        if (L.outHops) L.outHops[q] = st.hops;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int WR, int MAXCH>
+__global__ __launch_bounds__(256) void beamHnswKernel(HnswLaunch L) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = laneId();
+    const int waveInBlock = threadIdx.x >> 6;
+    const uint32_t wavesPerBlock = blockDim.x >> 6;
+    const uint32_t waveGlobal = blockIdx.x * wavesPerBlock + waveInBlock;
+    unsigned char* lds = smem + (size_t)waveInBlock * L.ldsPerWave;
+    uint32_t* gfilt = L.gfilt ? L.gfilt + ((size_t)waveGlobal << L.gbitsMax) : nullptr;
+
+    uint32_t q = waveGlobal;
+    if (L.queue) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(&L.queue[0], 1u);
+        q = (uint32_t)readlane((int)t, 0);
+    }
+    while (q < L.nq) {
+        int Ls = L.Ls, k = L.k;
+        int64_t visit = L.visit;
+        if (L.qp) {
+            Ls = L.qp[q].Ls;
+            k = L.qp[q].k;
+            visit = L.qp[q].visit_limit;
+        }
+        const int fb = hnswFilterBits(Ls);
+        if (Ls < 1 || Ls > L.LsMax || k < 0 || (uint32_t)k > L.outStride ||
+            (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
+            if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
+        } else if (fb <= L.fbitsLds) {
+            hnswOne<WR, MAXCH, false>(L, lds, nullptr, q, Ls, k, visit);
+        } else {
+            hnswOne<WR, MAXCH, true>(L, lds, gfilt, q, Ls, k, visit);
+        }
+        if (L.queue) {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(&L.queue[0], 1u);
+            q = (uint32_t)readlane((int)t, 0);
+        } else {
+            q += L.totalWaves;
+        }
+    }
+    if (L.queue && lane == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(&L.queue[1], 1u);
+        if (done + 1 == L.totalWaves) {
+            atomicExch(&L.queue[0], 0u);
+            atomicExch(&L.queue[1], 0u);
+        }
+    }
+}
+
+template <int WR, int MAXCH>
+hipError_t launchT(const HnswLaunch& L, int wpb, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL((beamHnswKernel<WR, MAXCH>), dim3(blocks), dim3(64 * wpb),
+                       (size_t)L.ldsPerWave * wpb, s, L);
+    return hipGetLastError();
+}
+
+template <int WR, int MAXCH>
+hipError_t occT(int wpb, size_t lds, int* bpc) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, beamHnswKernel<WR, MAXCH>, 64 * wpb, lds);
+}
+
+}  // namespace
+
+#define PANN_HNSW_DISPATCH(FN, ...)                                           \
+    switch (wr * 10 + maxch) {                                                \
+        case 11: return FN<1, 1>(__VA_ARGS__);                                \
+        case 12: return FN<1, 2>(__VA_ARGS__);                                \
+        case 14: return FN<1, 4>(__VA_ARGS__);                                \
+        case 21: return FN<2, 1>(__VA_ARGS__);                                \
+        case 22: return FN<2, 2>(__VA_ARGS__);                                \
+        case 24: return FN<2, 4>(__VA_ARGS__);                                \
+        case 41: return FN<4, 1>(__VA_ARGS__);                                \
+        case 42: return FN<4, 2>(__VA_ARGS__);                                \
+        case 44: return FN<4, 4>(__VA_ARGS__);                                \
+        case 81: return FN<8, 1>(__VA_ARGS__);                                \
+        case 82: return FN<8, 2>(__VA_ARGS__);                                \
+        case 84: return FN<8, 4>(__VA_ARGS__);                                \
+        default: return hipErrorInvalidValue;                                 \
+    }
+
+hipError_t launchBeamHnsw(const HnswLaunch& L, int wr, int maxch, int wavesPerBlock, int blocks,
+                          hipStream_t stream) {
+    PANN_HNSW_DISPATCH(launchT, L, wavesPerBlock, blocks, stream)
+}
+
+hipError_t beamHnswOccupancy(int wr, int maxch, int wavesPerBlock, size_t ldsBytes, int* bpc) {
+    PANN_HNSW_DISPATCH(occT, wavesPerBlock, ldsBytes, bpc)
+}
+
+}  // namespace pann

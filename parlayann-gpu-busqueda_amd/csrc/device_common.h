@@ -1,0 +1,255 @@
+// device_common.h -- wave64 building blocks shared by the search kernels (gfx950).
+#pragma once
+
+#include "pann_internal.h"
+
+namespace pann {
+namespace dev {
+
+// parlaylib hash64_2 (cuda/utiles_gpu.cuh:12-17): slot of ParlayANN's seen filter.
+__device__ __forceinline__ uint64_t hash64_2(uint64_t x) {
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+// parlaylib hash64 (cuda/utiles_gpu.cuh:21-31): HNSW marks its entry points with it.
+__device__ __forceinline__ uint64_t hash64(uint64_t u) {
+    uint64_t v = u * 3935559000370003845ull + 2691343689449507681ull;
+    v ^= v >> 21;
+    v ^= v << 37;
+    v ^= v >> 4;
+    v *= 4768777513237032717ull;
+    v ^= v << 20;
+    v ^= v >> 41;
+    v ^= v << 5;
+    return v;
+}
+
+__device__ __forceinline__ int laneId() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0));
+}
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shflXor64(uint64_t v, int m) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shflUp64(uint64_t v, int d) {
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, d, 64);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shflDown64(uint64_t v, int d) {
+    const uint32_t lo = (uint32_t)__shfl_down((int)(uint32_t)v, d, 64);
+    const uint32_t hi = (uint32_t)__shfl_down((int)(uint32_t)(v >> 32), d, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// DPP moves inside 8-lane groups.
+__device__ __forceinline__ int dppHalfMirror(int x) {  // lane i <- lane 7-i
+    return __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int dppXor2(int x) {  // quad_perm [2,3,0,1]
+    return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int dppXor1(int x) {  // quad_perm [1,0,3,2]
+    return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
+}
+
+// Sum of x over the 8-lane group, in every lane of the group.
+__device__ __forceinline__ int groupSum8(int x) {
+    x += dppXor1(x);
+    x += dppXor2(x);
+    x += dppHalfMirror(x);
+    return x;
+}
+
+// v[r] holds the lane's partial for round r.  Returns, in lane l, the 8-lane group total
+// of round (l & 7): a 3-step transpose-reduce (7 DPP moves instead of 24).
+__device__ __forceinline__ int transposeReduce8(const int (&v)[8], int lane) {
+    const bool b2 = lane & 4, b1 = lane & 2, b0 = lane & 1;
+    int w[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int send = b2 ? v[j] : v[j + 4];
+        const int keep = b2 ? v[j + 4] : v[j];
+        w[j] = keep + dppHalfMirror(send);
+    }
+    int x[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int send = b1 ? w[j] : w[j + 2];
+        const int keep = b1 ? w[j + 2] : w[j];
+        x[j] = keep + dppXor2(send);
+    }
+    const int send = b0 ? x[0] : x[1];
+    const int keep = b0 ? x[1] : x[0];
+    return keep + dppXor1(send);
+}
+
+__device__ __forceinline__ int sdot4(int a, int b, int c) {
+    return __builtin_amdgcn_sdot4(a, b, c, false);
+}
+
+// p.p - 2 q.p over one 16-byte slice (exact int32).
+__device__ __forceinline__ int slicePartial(const int4& v, const int4& q) {
+    int pp = sdot4(v.x, v.x, 0);
+    pp = sdot4(v.y, v.y, pp);
+    pp = sdot4(v.z, v.z, pp);
+    pp = sdot4(v.w, v.w, pp);
+    int qp = sdot4(v.x, q.x, 0);
+    qp = sdot4(v.y, q.y, qp);
+    qp = sdot4(v.z, q.z, qp);
+    qp = sdot4(v.w, q.w, qp);
+    return pp - 2 * qp;
+}
+
+__device__ __forceinline__ int sliceNorm(const int4& q) {
+    int s = sdot4(q.x, q.x, 0);
+    s = sdot4(q.y, q.y, s);
+    s = sdot4(q.z, q.z, s);
+    return sdot4(q.w, q.w, s);
+}
+
+// Distance from the query slice (8 lanes x 16 B) to one row, in every lane of the group.
+__device__ __forceinline__ int groupDistance(const int8_t* rows, uint32_t id, int sub,
+                                             const int4& qv, int qq) {
+    const int4 v = *reinterpret_cast<const int4*>(rows + (size_t)id * kVecBytes + sub * 16);
+    return qq + groupSum8(slicePartial(v, qv));
+}
+
+// Ascending bitonic sort of 64*C keys, element e = c*64 + lane in key[c].
+template <int C>
+__device__ __forceinline__ void bitonicSort(uint64_t (&key)[C], int lane) {
+    constexpr int N = 64 * C;
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    const int c2 = c ^ (j >> 6);
+                    if (c < c2) {
+                        const int e = c * 64 + lane;
+                        const bool asc = (e & k) == 0;
+                        const uint64_t a = key[c], b = key[c2];
+                        const bool sw = asc ? (a > b) : (a < b);
+                        key[c] = sw ? b : a;
+                        key[c2] = sw ? a : b;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    const int e = c * 64 + lane;
+                    const bool asc = (e & k) == 0;
+                    const bool lower = (lane & j) == 0;
+                    const uint64_t p = shflXor64(key[c], j);
+                    const uint64_t mn = key[c] < p ? key[c] : p;
+                    const uint64_t mx = key[c] < p ? p : key[c];
+                    key[c] = (lower == asc) ? mn : mx;
+                }
+            }
+        }
+    }
+}
+
+// Seen-filter accesses: LDS filters use plain ds ops; a per-wave HBM filter is read and
+// written at agent scope so the wave's own earlier writes are visible to it (the vector
+// L1 is not coherent with stores).
+template <bool G>
+__device__ __forceinline__ uint32_t filtLoad(const uint32_t* f, uint32_t h) {
+    if constexpr (G) return __hip_atomic_load(f + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return f[h];
+}
+template <bool G>
+__device__ __forceinline__ void filtStore(uint32_t* f, uint32_t h, uint32_t v) {
+    if constexpr (G) __hip_atomic_store(f + h, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else f[h] = v;
+}
+
+// Clears 2^bits filter slots to kEmpty.
+template <bool G>
+__device__ __forceinline__ void filtClear(uint32_t* f, int bits, int lane) {
+    const uint32_t slots = 1u << bits;
+    if constexpr (G) {
+        for (uint32_t i = lane; i < slots; i += 64) filtStore<true>(f, i, kEmpty);
+        __builtin_amdgcn_s_waitcnt(0);
+    } else {
+        if (slots >= 4) {
+            const uint4 e4 = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+            uint4* f4 = reinterpret_cast<uint4*>(f);
+            for (uint32_t i = lane; i < slots / 4; i += 64) f4[i] = e4;
+        } else if ((uint32_t)lane < slots) {
+            f[lane] = kEmpty;
+        }
+    }
+}
+
+// ParlayANN's lossy seen filter over one 64-entry chunk of an adjacency row, with the
+// sequential semantics of has_been_seen: entry a (participating) is "seen" iff the slot
+// held a just before its turn, then the slot holds a.  Slot reads happen for the whole
+// chunk at once; lanes whose byte tag was overwritten share a slot (or a tag) with
+// another lane and are resolved by a scalar loop over those groups.  Returns keep.
+template <bool G>
+__device__ __forceinline__ bool seenFilterChunk(uint32_t* filt, unsigned char* tag, uint32_t fmask,
+                                                uint32_t a, bool part, int lane,
+                                                uint64_t lanesBelow) {
+    const uint32_t h = (uint32_t)hash64_2(a) & fmask;
+    const uint32_t ts = h & (kTagSlots - 1);
+    if (part) tag[ts] = (unsigned char)lane;
+    __builtin_amdgcn_wave_barrier();
+    uint32_t prev = kEmpty;
+    bool coll = false;
+    if (part) {
+        coll = tag[ts] != (unsigned char)lane;
+        prev = filtLoad<G>(filt, h);
+    }
+    uint64_t cm = ballot(coll);
+    bool last = true;
+    while (cm) {
+        const int Lr = __ffsll((unsigned long long)cm) - 1;
+        const uint32_t hL = (uint32_t)readlane((int)h, Lr);
+        const uint64_t M = ballot(part && h == hL);
+        const bool inM = (M >> lane) & 1ull;
+        const uint64_t below = M & lanesBelow;
+        const uint64_t above = M & ~(lanesBelow | (1ull << lane));
+        const int src = (inM && below) ? (63 - __clzll((long long)below)) : lane;
+        const uint32_t x = (uint32_t)__shfl((int)a, src, 64);
+        if (inM && below) prev = x;
+        if (inM && above) last = false;
+        cm &= ~M;
+    }
+    const bool seen = part && (prev == a);
+    if (part && last) filtStore<G>(filt, h, a);
+    if constexpr (G) __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    return part && !seen;
+}
+
+}  // namespace dev
+}  // namespace pann

@@ -43,9 +43,11 @@ struct pann_searcher {
     pann_search_config cfg{};
     uint32_t nq = 0;
     int maxch = 1;
+    int wr = 1;
     int wavesPerBlock = 4;
     int blocks = 0;
     FlatLaunch L{};
+    HnswLaunch H{};
     uint32_t* gfilt = nullptr;
     pann_search_outputs out{};
 };
@@ -360,13 +362,23 @@ int pann_hnsw_create(uint32_t n, const uint32_t* level, const uint32_t* node_poi
             if (l == 0) G0 = std::max(G0, sz);
             else GA = std::max(GA, sz);
             const uint32_t* ids = list_ids + list_offsets[node_first_list[i] + l];
-            for (uint32_t j = 0; j < sz; j++)
+            for (uint32_t j = 0; j < sz; j++) {
                 if (ids[j] >= n) return setError(PANN_ERR_FORMAT, "neighbour %u out of range", ids[j]);
+                // a layer-l list may only name nodes present on layer l
+                if (level[ids[j]] < l)
+                    return setError(PANN_ERR_FORMAT, "node %u lists node %u on layer %u above its level %u",
+                                    i, ids[j], l, level[ids[j]]);
+            }
         }
         blocks += level[i];
     }
-    for (uint32_t e = 0; e < num_entries; e++)
+    for (uint32_t e = 0; e < num_entries; e++) {
         if (entries[e] >= n) return setError(PANN_ERR_FORMAT, "entry point out of range");
+        // every entry point starts the top-layer search, so it must live on the top layer
+        if (level[entries[e]] != level[entries[0]])
+            return setError(PANN_ERR_FORMAT, "entry point %u is on level %u, not the top level %u",
+                            entries[e], level[entries[e]], level[entries[0]]);
+    }
     if (G0 > 256 || GA > 256)
         return setError(PANN_ERR_CAPACITY, "degree %u/%u above the supported 256", G0, GA);
 
@@ -427,10 +439,11 @@ int pann_hnsw_create(uint32_t n, const uint32_t* level, const uint32_t* node_poi
 }
 
 int pann_hnsw_load(const char* path, const pann_vectors* base, pann_hnsw** out) {
-    if (!path || !base || !out) return setError(PANN_ERR_ARG, "NULL argument");
+    if (!path || !out) return setError(PANN_ERR_ARG, "NULL argument");
     HostHnsw h;
     int r = fromStatus(readHnsw(path, h));
     if (r != PANN_OK) return r;
+    if (!base) return setError(PANN_ERR_ARG, "base vectors are NULL");
     return pann_hnsw_create(h.n, h.level.data(), h.nodePoint.data(), h.firstList.data(),
                             h.listOffset.data(), h.listSize.data(), h.ids.data(),
                             h.entries.data(), (uint32_t)h.entries.size(), base, out);
@@ -554,20 +567,105 @@ int pann_searcher_launch(pann_searcher* s, const pann_vectors* base, const pann_
 
 int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_search_config* cfg,
                                pann_searcher** out) {
-    (void)h;
-    (void)nq;
-    (void)cfg;
-    if (out) *out = nullptr;
-    return setError(PANN_ERR_CAPACITY, "HNSW search kernel not available in this build");
+    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!h) return setError(PANN_ERR_ARG, "hnsw is NULL");
+    int r = checkConfig(cfg);
+    if (r != PANN_OK) return r;
+    const int Ls = cfg->Ls;
+    const int need = std::max<int>(Ls, (int)h->numEntries) + 9;
+    const int wr = need <= 64 ? 1 : need <= 128 ? 2 : need <= 256 ? 4 : need <= 512 ? 8 : 0;
+    if (!wr) return setError(PANN_ERR_CAPACITY, "HNSW Ls=%d above the supported 503", Ls);
+    const uint32_t G = std::max(h->G0, h->GA);
+    const int maxch = G <= 64 ? 1 : G <= 128 ? 2 : 4;
+    const int bits0 = hnswFilterBits(Ls);
+    const bool ldsFilter = bits0 <= kFilterBitsLds;
+    const uint32_t capInw = hnswInwCap(Ls);
+    const HnswLds lay = hnswLdsLayout(maxch, capInw, ldsFilter ? bits0 : 0);
+    if (lay.total > kMaxLdsPerBlock)
+        return setError(PANN_ERR_CAPACITY, "Ls=%d needs %u bytes of LDS per query", Ls, lay.total);
+
+    pann_searcher* s = new pann_searcher();
+    s->kind = 1;
+    s->cfg = *cfg;
+    s->nq = nq;
+    s->maxch = maxch;
+    s->wr = wr;
+    s->wavesPerBlock = (int)std::min<size_t>(4, std::max<size_t>(1, kMaxLdsPerBlock / lay.total));
+    int bpc = 0, cus = 0;
+    hipError_t he = beamHnswOccupancy(wr, maxch, s->wavesPerBlock, (size_t)lay.total * s->wavesPerBlock, &bpc);
+    r = deviceCUs(&cus);
+    if (he != hipSuccess || r != PANN_OK) {
+        delete s;
+        return he != hipSuccess ? setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(he)) : r;
+    }
+    const uint32_t want = (nq + s->wavesPerBlock - 1) / s->wavesPerBlock;
+    s->blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>(want, (uint32_t)std::max(1, bpc) * cus));
+    const uint32_t totalWaves = (uint32_t)s->blocks * s->wavesPerBlock;
+    HnswLaunch& H = s->H;
+    H.n = h->n;
+    H.G0 = h->G0;
+    H.GA = h->GA;
+    H.top = (int32_t)h->top;
+    H.numEntries = (int32_t)h->numEntries;
+    for (int e = 0; e < 8; e++) H.entries[e] = e < (int)h->numEntries ? h->entries[e] : 0;
+    H.nq = nq;
+    H.qp = nullptr;
+    H.Ls = Ls;
+    H.k = cfg->k;
+    H.visit = cfg->visit_limit;
+    H.LsMax = Ls;
+    H.fbitsLds = ldsFilter ? bits0 : 2;
+    H.gbitsMax = ldsFilter ? 0 : bits0;
+    H.capInw = capInw;
+    H.ldsPerWave = lay.total;
+    H.lds = lay;
+    H.queue = nullptr;
+    H.totalWaves = totalWaves;
+    if (!ldsFilter) {
+        he = hipMalloc(&s->gfilt, ((size_t)totalWaves << bits0) * 4);
+        if (he != hipSuccess) {
+            delete s;
+            return setError(PANN_ERR_HIP, "allocating filter scratch: %s", hipGetErrorString(he));
+        }
+    }
+    H.gfilt = s->gfilt;
+    r = allocOutputs(s->out, nq, (uint32_t)std::max(1, cfg->k));
+    if (r != PANN_OK) {
+        pann_searcher_free(s);
+        return r;
+    }
+    H.outIds = s->out.ids;
+    H.outDists = s->out.dists;
+    H.outStride = s->out.stride;
+    H.outComps = s->out.comps;
+    H.outFlags = s->out.flags;
+    H.outEdges = s->out.edges;
+    H.outHops = s->out.hops;
+    *out = s;
+    return PANN_OK;
 }
 
 int pann_hnsw_searcher_launch(pann_searcher* s, const pann_vectors* queries, const pann_hnsw* h,
                               void* stream) {
-    (void)s;
-    (void)queries;
-    (void)h;
-    (void)stream;
-    return setError(PANN_ERR_CAPACITY, "HNSW search kernel not available in this build");
+    if (!s || !queries || !h) return setError(PANN_ERR_ARG, "NULL argument");
+    if (s->kind != 1) return setError(PANN_ERR_ARG, "searcher was prepared for a flat graph");
+    if (h->n != s->H.n || h->G0 != s->H.G0 || h->GA != s->H.GA)
+        return setError(PANN_ERR_ARG, "index differs from the prepared one");
+    if (queries->dim != h->dim)
+        return setError(PANN_ERR_ARG, "dim mismatch index %u queries %u", h->dim, queries->dim);
+    if (queries->n < s->nq) return setError(PANN_ERR_ARG, "%u queries, searcher prepared for %u", queries->n, s->nq);
+    if (s->nq == 0) return PANN_OK;
+    HnswLaunch H = s->H;
+    H.vec = h->nodeVec;
+    H.adj0 = h->adj0;
+    H.adjUp = h->adjUp;
+    H.upStart = h->upStart;
+    H.nodePoint = h->nodePoint;
+    H.queries = queries->dev;
+    hipError_t e = launchBeamHnsw(H, s->wr, s->maxch, s->wavesPerBlock, s->blocks, (hipStream_t)stream);
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "HNSW search launch: %s", hipGetErrorString(e));
+    return PANN_OK;
 }
 
 int pann_searcher_outputs(const pann_searcher* s, pann_search_outputs* out) {

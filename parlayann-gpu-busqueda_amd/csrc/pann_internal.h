@@ -109,6 +109,64 @@ struct FlatLaunch {
     int32_t* outHops;
 };
 
+// ---- HNSW ----
+// filter bits of HNSW::search at ef: ef > 2 ? 2*ceil(log2 ef) - 2 : 2   cuda/busqueda_hnsw.cu:556-557
+__host__ __device__ inline int hnswFilterBits(int ef) { return ef > 2 ? 2 * ceilLog2(ef) - 2 : 2; }
+// capacity of the exact "entered the workset" set of one layer search
+__host__ __device__ inline uint32_t hnswInwCap(int Ls) { return 1u << ceilLog2((uint64_t)16 * Ls + 256); }
+
+struct HnswLds {
+    uint32_t pend;   // pending ids (64*maxch u32)
+    uint32_t inw;    // is_inw hash (capInw u32)
+    uint32_t filt;   // seen filter when in LDS (max(4, 1 << fbitsLds) u32)
+    uint32_t tag;    // kTagSlots u8
+    uint32_t total;
+};
+
+__host__ __device__ inline HnswLds hnswLdsLayout(int maxch, uint32_t capInw, int fbitsLds) {
+    HnswLds L;
+    uint32_t o = 0;
+    L.pend = o; o += 4u * 64u * (uint32_t)maxch;
+    L.inw = o;  o += 4u * capInw;
+    L.filt = o; o += 4u * (fbitsLds > 2 ? (1u << fbitsLds) : 4u);
+    L.tag = o;  o += kTagSlots;
+    L.total = alignUp(o, 16);
+    return L;
+}
+
+struct HnswLaunch {
+    const int8_t* vec;         // vectors in node order, kVecBytes pitch
+    const uint32_t* adj0;      // n x G0
+    const uint32_t* adjUp;     // blocks x GA
+    const uint32_t* upStart;   // n: first upper block of a node
+    const uint32_t* nodePoint; // n: base point held by a node
+    uint32_t n, G0, GA;
+    int32_t top, numEntries;
+    uint32_t entries[8];
+    const int8_t* queries;
+    uint32_t nq;
+    const QueryParams* qp;     // nullable: uniform parameters below
+    int32_t Ls, k;
+    int64_t visit;             // <= 0: none
+    int32_t LsMax, fbitsLds, gbitsMax;
+    uint32_t capInw, ldsPerWave;
+    HnswLds lds;
+    uint32_t* gfilt;
+    uint32_t* queue;
+    uint32_t totalWaves;
+    uint32_t* outIds;
+    int32_t* outDists;
+    uint32_t outStride;
+    int32_t* outComps;
+    int32_t* outFlags;
+    int32_t* outEdges;
+    int32_t* outHops;
+};
+
+hipError_t launchBeamHnsw(const HnswLaunch& L, int wr, int maxch, int wavesPerBlock, int blocks,
+                          hipStream_t stream);
+hipError_t beamHnswOccupancy(int wr, int maxch, int wavesPerBlock, size_t ldsBytes, int* blocksPerCU);
+
 // Launchers implemented in the .hip translation units.
 hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int blocks,
                           hipStream_t stream);
