@@ -79,6 +79,14 @@ void parallelFor(size_t n, int nthreads, F&& body, size_t chunk = 16) {
 int writeParlayGraph(const char* path, uint32_t n, uint32_t maxDeg, const uint32_t* degrees,
                      const uint32_t* dense, uint32_t stride);
 
+// The search always starts at node 0.  For every node not reachable from 0 (in id order),
+// adds one edge from its nearest reachable node, then marks what became reachable.
+// Returns the number of edges added.
+size_t bridgeFromStart(const int8_t* base, uint32_t n, uint32_t dim,
+                       std::vector<std::vector<uint32_t>>& adj, int nthreads);
+
+int writeGraphLists(const char* path, const std::vector<std::vector<uint32_t>>& adj);
+
 struct Nbr {
     int32_t d;
     uint32_t id;
