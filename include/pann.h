@@ -161,6 +161,47 @@ int pann_searcher_download(const pann_searcher* s, uint32_t* ids, int32_t* dists
                            int32_t* flags, int32_t* edges, int32_t* hops, void* stream);
 void pann_searcher_free(pann_searcher* s); /* liberar */
 
+/* ---- batched persistent form (no counterpart in the reference, which launches one
+ *      configuration at a time, cuda/comparacion1.cu:224-247) ---- */
+typedef struct pann_batch pann_batch;
+
+/* Sizing of a batch.  Queries may each ask for any Ls <= max_Ls and k <= max_k. */
+typedef struct pann_batch_config {
+    int32_t max_Ls;          /* largest per-query Ls */
+    int32_t max_k;           /* output stride */
+    int32_t waves_per_block; /* 1, 2 or 4 (0: the most that fit 64 KB of LDS) */
+    int32_t lds_filter_bits; /* seen filters up to 2^bits slots live in LDS, 10..12 (0: 12) */
+    uint32_t visited_slots;  /* LDS visited / is_inw hash slots, power of two (0: derived) */
+    uint32_t reserved;
+} pann_batch_config;
+
+/* Allocates device buffers for up to max_queries queries over a flat graph or HNSW. */
+int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint32_t max_queries,
+                            const pann_batch_config* cfg, pann_batch** out);
+int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann_batch_config* cfg,
+                            pann_batch** out);
+/* Uploads nq host query rows (dim bytes each) and their parameters; the queue hands out
+ * queries longest-Ls first.  Asynchronous on `stream` (host arrays must stay valid until
+ * it completes). */
+int pann_batch_set_queries(pann_batch* b, const int8_t* host_rows, const pann_query_params* params,
+                           uint32_t nq, void* stream);
+/* One persistent launch whose resident wavefronts drain the queue. */
+int pann_batch_launch(pann_batch* b, void* stream);
+/* Records that launch into a HIP graph (for the current nq) / replays it. */
+int pann_batch_capture(pann_batch* b, void* stream);
+int pann_batch_replay(pann_batch* b, void* stream);
+int pann_batch_outputs(const pann_batch* b, pann_search_outputs* out);
+int pann_batch_download(const pann_batch* b, uint32_t* ids, int32_t* dists, int32_t* comps,
+                        int32_t* flags, int32_t* edges, int32_t* hops, void* stream);
+int pann_batch_info(const pann_batch* b, pann_batch_config* cfg, uint32_t* resident_waves);
+void pann_batch_free(pann_batch* b);
+/* Autotuner: times every candidate configuration (wavefronts per block, LDS filter size,
+ * visited-hash slots) on the given sample and writes the fastest whose runs raised no
+ * overflow flag into *best.  g or h selects the index (the other is NULL). */
+int pann_batch_autotune(const pann_vectors* base, const pann_graph* g, const pann_hnsw* h,
+                        const int8_t* host_rows, const pann_query_params* params, uint32_t nq,
+                        int32_t max_Ls, int32_t max_k, pann_batch_config* best, float* best_ms);
+
 /* ---- exact ground truth ---- */
 /* Exact top-k (k <= 128) of the first nq queries over all base rows, ordered by
  * (distance, id), written to device arrays ids/dists (nq x k).  int8 -> int32 MFMA

@@ -340,12 +340,7 @@ __global__ __launch_bounds__(256) void beamHnswKernel(HnswLaunch L) {
     unsigned char* lds = smem + (size_t)waveInBlock * L.ldsPerWave;
     uint32_t* gfilt = L.gfilt ? L.gfilt + ((size_t)waveGlobal << L.gbitsMax) : nullptr;
 
-    uint32_t q = waveGlobal;
-    if (L.queue) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(&L.queue[0], 1u);
-        q = (uint32_t)readlane((int)t, 0);
-    }
+    uint32_t q = L.queue ? takeQuery(L.queue, L.order, L.nq, lane) : waveGlobal;
     while (q < L.nq) {
         int Ls = L.Ls, k = L.k;
         int64_t visit = L.visit;
@@ -363,22 +358,9 @@ __global__ __launch_bounds__(256) void beamHnswKernel(HnswLaunch L) {
         } else {
             hnswOne<WR, MAXCH, true>(L, lds, gfilt, q, Ls, k, visit);
         }
-        if (L.queue) {
-            uint32_t t = 0;
-            if (lane == 0) t = atomicAdd(&L.queue[0], 1u);
-            q = (uint32_t)readlane((int)t, 0);
-        } else {
-            q += L.totalWaves;
-        }
+        q = L.queue ? takeQuery(L.queue, L.order, L.nq, lane) : q + L.totalWaves;
     }
-    if (L.queue && lane == 0) {
-        __threadfence();
-        const uint32_t done = atomicAdd(&L.queue[1], 1u);
-        if (done + 1 == L.totalWaves) {
-            atomicExch(&L.queue[0], 0u);
-            atomicExch(&L.queue[1], 0u);
-        }
-    }
+    if (L.queue) finishQueue(L.queue, L.totalWaves, lane);
 }
 
 template <int WR, int MAXCH>

@@ -251,5 +251,29 @@ __device__ __forceinline__ bool seenFilterChunk(uint32_t* filt, unsigned char* t
     return part && !seen;
 }
 
+// Persistent work queue: queue[0] is the next ticket, queue[1] counts finished waves.
+// Tickets map to queries through `order` (NULL: identity); nq means "drained".
+__device__ __forceinline__ uint32_t takeQuery(uint32_t* queue, const uint32_t* order, uint32_t nq,
+                                              int lane) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&queue[0], 1u);
+    t = (uint32_t)readlane((int)t, 0);
+    if (t >= nq) return nq;
+    return order ? order[t] : t;
+}
+
+// The last wave to finish resets the queue, so the same launch (or a captured graph of
+// it) can run again without a host-side clear.
+__device__ __forceinline__ void finishQueue(uint32_t* queue, uint32_t totalWaves, int lane) {
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(&queue[1], 1u);
+        if (done + 1 == totalWaves) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+        }
+    }
+}
+
 }  // namespace dev
 }  // namespace pann

@@ -10,49 +10,11 @@
 #include <vector>
 
 #include "formats.h"
-#include "pann_internal.h"
+#include "pann_objects.h"
 
 using namespace pann;
 
-struct pann_vectors {
-    uint32_t n = 0, dim = 0;
-    int8_t* dev = nullptr;  // n x kVecBytes
-};
-
-struct pann_graph {
-    uint32_t n = 0, G = 0;
-    uint32_t* adj = nullptr;  // n x G
-};
-
-struct pann_hnsw {
-    uint32_t n = 0;
-    uint32_t G0 = 0, GA = 0;
-    uint32_t top = 0, numEntries = 0;
-    uint32_t entries[8] = {};
-    uint32_t* adj0 = nullptr;       // n x G0
-    uint32_t* adjUp = nullptr;      // blocks x GA
-    uint32_t* upStart = nullptr;    // n
-    uint32_t* nodePoint = nullptr;  // n
-    int8_t* nodeVec = nullptr;      // n x kVecBytes, vectors in node order
-    bool ownVec = false;
-    uint32_t dim = 0;
-};
-
-struct pann_searcher {
-    int kind = 0;  // 0 flat, 1 hnsw
-    pann_search_config cfg{};
-    uint32_t nq = 0;
-    int maxch = 1;
-    int wr = 1;
-    int wavesPerBlock = 4;
-    int blocks = 0;
-    FlatLaunch L{};
-    HnswLaunch H{};
-    uint32_t* gfilt = nullptr;
-    pann_search_outputs out{};
-};
-
-namespace {
+namespace pann {
 
 thread_local std::string g_lastError;
 
@@ -71,13 +33,6 @@ int fromStatus(const Status& s) {
     g_lastError = s.msg;
     return s.code;
 }
-
-#define HIP_TRY(call)                                                                      \
-    do {                                                                                   \
-        hipError_t e_ = (call);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            return setError(PANN_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
-    } while (0)
 
 int deviceCUs(int* cus) {
     int dev = 0;
@@ -130,11 +85,7 @@ int checkConfig(const pann_search_config* cfg) {
     return PANN_OK;
 }
 
-// Largest filter kept in LDS (2^12 u32 = 16 KB); larger ones use a per-wave HBM region.
-constexpr int kFilterBitsLds = 12;
-constexpr size_t kMaxLdsPerBlock = 64 * 1024;
-
-}  // namespace
+}  // namespace pann
 
 extern "C" {
 

@@ -1,0 +1,406 @@
+// pann_batch.cpp -- the batched persistent form: one launch of resident wavefronts drains
+// a device queue of queries that each carry their own qid, Ls, k, visit limit and cut;
+// the launch can be captured into a HIP graph and replayed; an autotuner picks the
+// launch shape per index.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "pann_objects.h"
+
+using namespace pann;
+
+struct pann_batch {
+    int kind = 0;  // 0 flat, 1 hnsw
+    const pann_vectors* base = nullptr;
+    const pann_graph* graph = nullptr;
+    const pann_hnsw* hnsw = nullptr;
+    pann_batch_config cfg{};
+    uint32_t maxQueries = 0, nq = 0, dim = 0;
+    int maxch = 1, wr = 1, wavesPerBlock = 1, blocks = 0;
+    uint32_t totalWaves = 0;
+    FlatLaunch L{};
+    HnswLaunch H{};
+    int8_t* queries = nullptr;
+    QueryParams* params = nullptr;
+    uint32_t* order = nullptr;
+    uint32_t* queue = nullptr;
+    uint32_t* gfilt = nullptr;
+    pann_search_outputs out{};
+    std::vector<uint32_t> hostOrder;
+    hipGraph_t graphDef = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    uint32_t capturedNq = 0;
+};
+
+namespace {
+
+void releaseGraph(pann_batch* b) {
+    if (b->graphExec) hipGraphExecDestroy(b->graphExec);
+    if (b->graphDef) hipGraphDestroy(b->graphDef);
+    b->graphExec = nullptr;
+    b->graphDef = nullptr;
+    b->capturedNq = 0;
+}
+
+int checkBatchConfig(const pann_batch_config* cfg) {
+    if (!cfg) return setError(PANN_ERR_ARG, "config is NULL");
+    if (cfg->max_Ls < 1) return setError(PANN_ERR_ARG, "max_Ls must be >= 1");
+    if (cfg->max_k < 1) return setError(PANN_ERR_ARG, "max_k must be >= 1");
+    if (cfg->waves_per_block < 0 || cfg->waves_per_block > 4 || cfg->waves_per_block == 3)
+        return setError(PANN_ERR_ARG, "waves_per_block must be 0, 1, 2 or 4");
+    if (cfg->lds_filter_bits != 0 && (cfg->lds_filter_bits < 2 || cfg->lds_filter_bits > 12))
+        return setError(PANN_ERR_ARG, "lds_filter_bits must be 0 or 2..12");
+    if (cfg->visited_slots && (cfg->visited_slots & (cfg->visited_slots - 1)))
+        return setError(PANN_ERR_ARG, "visited_slots must be a power of two");
+    return PANN_OK;
+}
+
+// common allocation of query/params/order/queue/output buffers
+int allocCommon(pann_batch* b) {
+    const size_t mq = std::max<uint32_t>(1, b->maxQueries);
+    HIP_TRY(hipMalloc(&b->queries, mq * kVecBytes));
+    HIP_TRY(hipMemset(b->queries, 0, mq * kVecBytes));
+    HIP_TRY(hipMalloc(&b->params, mq * sizeof(QueryParams)));
+    HIP_TRY(hipMalloc(&b->order, mq * 4));
+    HIP_TRY(hipMalloc(&b->queue, 2 * 4));
+    HIP_TRY(hipMemset(b->queue, 0, 2 * 4));
+    int r = allocOutputs(b->out, b->maxQueries, (uint32_t)b->cfg.max_k);
+    if (r != PANN_OK) return r;
+    return PANN_OK;
+}
+
+int finishSizing(pann_batch* b, uint32_t ldsPerWave, int gbits) {
+    int cus = 0;
+    int r = deviceCUs(&cus);
+    if (r != PANN_OK) return r;
+    int wpb = b->cfg.waves_per_block;
+    if (wpb == 0) wpb = (int)std::min<size_t>(4, std::max<size_t>(1, kMaxLdsPerBlock / ldsPerWave));
+    if ((size_t)wpb * ldsPerWave > kMaxLdsPerBlock)
+        return setError(PANN_ERR_CAPACITY, "%d wavefronts x %u bytes of LDS exceed 64 KB", wpb, ldsPerWave);
+    b->wavesPerBlock = wpb;
+    b->cfg.waves_per_block = wpb;
+    int bpc = 0;
+    hipError_t e = b->kind == 0
+                       ? beamFlatOccupancy(b->maxch, wpb, (size_t)ldsPerWave * wpb, &bpc)
+                       : beamHnswOccupancy(b->wr, b->maxch, wpb, (size_t)ldsPerWave * wpb, &bpc);
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
+    b->blocks = std::max(1, bpc) * cus;
+    b->totalWaves = (uint32_t)b->blocks * wpb;
+    if (gbits > 0) HIP_TRY(hipMalloc(&b->gfilt, ((size_t)b->totalWaves << gbits) * 4));
+    return allocCommon(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint32_t max_queries,
+                            const pann_batch_config* cfg, pann_batch** out) {
+    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!base || !g) return setError(PANN_ERR_ARG, "NULL index");
+    if (base->n != g->n) return setError(PANN_ERR_ARG, "base has %u rows, graph %u nodes", base->n, g->n);
+    int r = checkBatchConfig(cfg);
+    if (r != PANN_OK) return r;
+    pann_batch* b = new pann_batch();
+    b->kind = 0;
+    b->base = base;
+    b->graph = g;
+    b->cfg = *cfg;
+    b->maxQueries = max_queries;
+    b->dim = base->dim;
+    b->maxch = g->G <= 64 ? 1 : g->G <= 128 ? 2 : 4;
+    const int LsMax = cfg->max_Ls;
+    const int fbMax = flatFilterBits(LsMax);
+    const int fbLds = std::min(cfg->lds_filter_bits ? cfg->lds_filter_bits : kFilterBitsLds, fbMax);
+    const uint32_t capV = cfg->visited_slots ? cfg->visited_slots : flatVisitedCap(LsMax);
+    b->cfg.lds_filter_bits = fbLds;
+    b->cfg.visited_slots = capV;
+    const FlatLds lay = flatLdsLayout(LsMax, b->maxch, capV, fbLds);
+    if (lay.total > kMaxLdsPerBlock) {
+        delete b;
+        return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsMax, lay.total);
+    }
+    const int gbits = fbMax > fbLds ? fbMax : 0;
+    r = finishSizing(b, lay.total, gbits);
+    if (r != PANN_OK) {
+        pann_batch_free(b);
+        return r;
+    }
+    FlatLaunch& L = b->L;
+    L.n = g->n;
+    L.G = g->G;
+    L.LsMax = LsMax;
+    L.fbitsLds = fbLds;
+    L.gbitsMax = gbits;
+    L.capV = capV;
+    L.ldsPerWave = lay.total;
+    L.lds = lay;
+    L.gfilt = b->gfilt;
+    L.queue = b->queue;
+    L.order = b->order;
+    L.totalWaves = b->totalWaves;
+    L.qp = b->params;
+    L.qids = nullptr;
+    L.queries = b->queries;
+    L.base = base->dev;
+    L.adj = g->adj;
+    L.outIds = b->out.ids;
+    L.outDists = b->out.dists;
+    L.outStride = b->out.stride;
+    L.outComps = b->out.comps;
+    L.outFlags = b->out.flags;
+    L.outEdges = b->out.edges;
+    L.outHops = b->out.hops;
+    *out = b;
+    return PANN_OK;
+}
+
+int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann_batch_config* cfg,
+                            pann_batch** out) {
+    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!h) return setError(PANN_ERR_ARG, "NULL index");
+    int r = checkBatchConfig(cfg);
+    if (r != PANN_OK) return r;
+    const int LsMax = cfg->max_Ls;
+    const int need = std::max<int>(LsMax, (int)h->numEntries) + 9;
+    const int wr = need <= 64 ? 1 : need <= 128 ? 2 : need <= 256 ? 4 : need <= 512 ? 8 : 0;
+    if (!wr) return setError(PANN_ERR_CAPACITY, "HNSW max_Ls=%d above the supported 503", LsMax);
+    pann_batch* b = new pann_batch();
+    b->kind = 1;
+    b->hnsw = h;
+    b->cfg = *cfg;
+    b->maxQueries = max_queries;
+    b->dim = h->dim;
+    b->wr = wr;
+    const uint32_t G = std::max(h->G0, h->GA);
+    b->maxch = G <= 64 ? 1 : G <= 128 ? 2 : 4;
+    const int fbMax = hnswFilterBits(LsMax);
+    const int fbLds = std::max(2, std::min(cfg->lds_filter_bits ? cfg->lds_filter_bits : kFilterBitsLds, fbMax));
+    const uint32_t capInw = cfg->visited_slots ? cfg->visited_slots : hnswInwCap(LsMax);
+    if (capInw < 256u * b->maxch) {
+        delete b;
+        return setError(PANN_ERR_CAPACITY, "visited_slots %u too small for rows of %u", capInw, G);
+    }
+    b->cfg.lds_filter_bits = fbLds;
+    b->cfg.visited_slots = capInw;
+    const HnswLds lay = hnswLdsLayout(b->maxch, capInw, fbLds);
+    if (lay.total > kMaxLdsPerBlock) {
+        delete b;
+        return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsMax, lay.total);
+    }
+    const int gbits = fbMax > fbLds ? fbMax : 0;
+    r = finishSizing(b, lay.total, gbits);
+    if (r != PANN_OK) {
+        pann_batch_free(b);
+        return r;
+    }
+    HnswLaunch& H = b->H;
+    H.vec = h->nodeVec;
+    H.adj0 = h->adj0;
+    H.adjUp = h->adjUp;
+    H.upStart = h->upStart;
+    H.nodePoint = h->nodePoint;
+    H.n = h->n;
+    H.G0 = h->G0;
+    H.GA = h->GA;
+    H.top = (int32_t)h->top;
+    H.numEntries = (int32_t)h->numEntries;
+    for (int e = 0; e < 8; e++) H.entries[e] = e < (int)h->numEntries ? h->entries[e] : 0;
+    H.queries = b->queries;
+    H.qp = b->params;
+    H.LsMax = LsMax;
+    H.fbitsLds = fbLds;
+    H.gbitsMax = gbits;
+    H.capInw = capInw;
+    H.ldsPerWave = lay.total;
+    H.lds = lay;
+    H.gfilt = b->gfilt;
+    H.queue = b->queue;
+    H.order = b->order;
+    H.totalWaves = b->totalWaves;
+    H.outIds = b->out.ids;
+    H.outDists = b->out.dists;
+    H.outStride = b->out.stride;
+    H.outComps = b->out.comps;
+    H.outFlags = b->out.flags;
+    H.outEdges = b->out.edges;
+    H.outHops = b->out.hops;
+    *out = b;
+    return PANN_OK;
+}
+
+int pann_batch_set_queries(pann_batch* b, const int8_t* host_rows, const pann_query_params* params,
+                           uint32_t nq, void* stream) {
+    if (!b || (nq && (!host_rows || !params))) return setError(PANN_ERR_ARG, "NULL argument");
+    if (nq > b->maxQueries) return setError(PANN_ERR_CAPACITY, "%u queries, batch holds %u", nq, b->maxQueries);
+    for (uint32_t i = 0; i < nq; i++) {
+        if (params[i].Ls < 1 || params[i].Ls > b->cfg.max_Ls)
+            return setError(PANN_ERR_ARG, "query %u: Ls=%d outside 1..%d", i, params[i].Ls, b->cfg.max_Ls);
+        if (params[i].k < 0 || params[i].k > b->cfg.max_k)
+            return setError(PANN_ERR_ARG, "query %u: k=%d outside 0..%d", i, params[i].k, b->cfg.max_k);
+        if (!(params[i].cut > 0)) return setError(PANN_ERR_ARG, "query %u: cut must be > 0", i);
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    b->nq = nq;
+    if (nq == 0) return PANN_OK;
+    // longest beams first: the persistent wavefronts then finish close together
+    b->hostOrder.resize(nq);
+    std::iota(b->hostOrder.begin(), b->hostOrder.end(), 0u);
+    std::stable_sort(b->hostOrder.begin(), b->hostOrder.end(),
+                     [&](uint32_t x, uint32_t y) { return params[x].Ls > params[y].Ls; });
+    HIP_TRY(hipMemcpy2DAsync(b->queries, kVecBytes, host_rows, b->dim, b->dim, nq, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b->params, params, (size_t)nq * sizeof(QueryParams), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b->order, b->hostOrder.data(), (size_t)nq * 4, hipMemcpyHostToDevice, st));
+    if (!stream) HIP_TRY(hipStreamSynchronize(nullptr));
+    return PANN_OK;
+}
+
+int pann_batch_launch(pann_batch* b, void* stream) {
+    if (!b) return setError(PANN_ERR_ARG, "batch is NULL");
+    if (b->nq == 0) return PANN_OK;
+    hipError_t e;
+    if (b->kind == 0) {
+        FlatLaunch L = b->L;
+        L.nq = b->nq;
+        e = launchBeamFlat(L, b->maxch, b->wavesPerBlock, b->blocks, (hipStream_t)stream);
+    } else {
+        HnswLaunch H = b->H;
+        H.nq = b->nq;
+        e = launchBeamHnsw(H, b->wr, b->maxch, b->wavesPerBlock, b->blocks, (hipStream_t)stream);
+    }
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "batched launch: %s", hipGetErrorString(e));
+    return PANN_OK;
+}
+
+int pann_batch_capture(pann_batch* b, void* stream) {
+    if (!b) return setError(PANN_ERR_ARG, "batch is NULL");
+    if (!stream) return setError(PANN_ERR_ARG, "capture needs a non-default stream");
+    releaseGraph(b);
+    const hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int r = pann_batch_launch(b, stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (r != PANN_OK) {
+        if (g) hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "end capture: %s", hipGetErrorString(e));
+    b->graphDef = g;
+    HIP_TRY(hipGraphInstantiate(&b->graphExec, g, nullptr, nullptr, 0));
+    b->capturedNq = b->nq;
+    return PANN_OK;
+}
+
+int pann_batch_replay(pann_batch* b, void* stream) {
+    if (!b) return setError(PANN_ERR_ARG, "batch is NULL");
+    if (!b->graphExec) return setError(PANN_ERR_ARG, "no captured launch");
+    if (b->capturedNq != b->nq) return setError(PANN_ERR_ARG, "queries changed since the capture");
+    HIP_TRY(hipGraphLaunch(b->graphExec, (hipStream_t)stream));
+    return PANN_OK;
+}
+
+int pann_batch_outputs(const pann_batch* b, pann_search_outputs* out) {
+    if (!b || !out) return setError(PANN_ERR_ARG, "NULL argument");
+    *out = b->out;
+    out->nq = b->nq;
+    return PANN_OK;
+}
+
+int pann_batch_download(const pann_batch* b, uint32_t* ids, int32_t* dists, int32_t* comps,
+                        int32_t* flags, int32_t* edges, int32_t* hops, void* stream) {
+    if (!b) return setError(PANN_ERR_ARG, "batch is NULL");
+    const hipStream_t st = (hipStream_t)stream;
+    const size_t nk = (size_t)b->nq * b->out.stride * 4, nn = (size_t)b->nq * 4;
+    if (b->nq == 0) return PANN_OK;
+    if (ids) HIP_TRY(hipMemcpyAsync(ids, b->out.ids, nk, hipMemcpyDeviceToHost, st));
+    if (dists) HIP_TRY(hipMemcpyAsync(dists, b->out.dists, nk, hipMemcpyDeviceToHost, st));
+    if (comps) HIP_TRY(hipMemcpyAsync(comps, b->out.comps, nn, hipMemcpyDeviceToHost, st));
+    if (flags) HIP_TRY(hipMemcpyAsync(flags, b->out.flags, nn, hipMemcpyDeviceToHost, st));
+    if (edges) HIP_TRY(hipMemcpyAsync(edges, b->out.edges, nn, hipMemcpyDeviceToHost, st));
+    if (hops) HIP_TRY(hipMemcpyAsync(hops, b->out.hops, nn, hipMemcpyDeviceToHost, st));
+    if (!stream) HIP_TRY(hipStreamSynchronize(nullptr));
+    return PANN_OK;
+}
+
+int pann_batch_info(const pann_batch* b, pann_batch_config* cfg, uint32_t* resident_waves) {
+    if (!b) return setError(PANN_ERR_ARG, "batch is NULL");
+    if (cfg) *cfg = b->cfg;
+    if (resident_waves) *resident_waves = b->totalWaves;
+    return PANN_OK;
+}
+
+void pann_batch_free(pann_batch* b) {
+    if (!b) return;
+    releaseGraph(b);
+    hipFree(b->queries);
+    hipFree(b->params);
+    hipFree(b->order);
+    hipFree(b->queue);
+    hipFree(b->gfilt);
+    freeOutputs(b->out);
+    delete b;
+}
+
+int pann_batch_autotune(const pann_vectors* base, const pann_graph* g, const pann_hnsw* h,
+                        const int8_t* host_rows, const pann_query_params* params, uint32_t nq,
+                        int32_t max_Ls, int32_t max_k, pann_batch_config* best, float* best_ms) {
+    if (!best || (!g && !h) || (g && !base) || !host_rows || !params || nq == 0)
+        return setError(PANN_ERR_ARG, "invalid autotune arguments");
+    const uint32_t derived = h ? hnswInwCap(max_Ls) : flatVisitedCap(max_Ls);
+    const int fbMax = h ? hnswFilterBits(max_Ls) : flatFilterBits(max_Ls);
+    std::vector<pann_batch_config> cands;
+    for (int wpb : {1, 2, 4})
+        for (int fb : {10, 11, 12}) {
+            if (fb > std::max(fbMax, 10)) continue;
+            for (uint32_t vs : {derived, derived / 2, derived * 2}) {
+                if (vs < (h ? 512u : 256u) || vs > 16384) continue;
+                if (!h && vs < 2u * (uint32_t)max_Ls + 64) continue;
+                cands.push_back({max_Ls, max_k, wpb, fb, vs, 0});
+            }
+        }
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float bestMs = 1e30f;
+    bool found = false;
+    std::vector<int32_t> flags(nq);
+    for (const auto& c : cands) {
+        pann_batch* b = nullptr;
+        const int r = h ? pann_batch_prepare_hnsw(h, nq, &c, &b) : pann_batch_prepare_flat(base, g, nq, &c, &b);
+        if (r != PANN_OK) continue;  // does not fit: skip this shape
+        float ms = 1e30f;
+        bool ok = pann_batch_set_queries(b, host_rows, params, nq, st) == PANN_OK &&
+                  pann_batch_launch(b, st) == PANN_OK;
+        for (int rep = 0; ok && rep < 3; rep++) {
+            ok = hipEventRecord(e0, st) == hipSuccess && pann_batch_launch(b, st) == PANN_OK &&
+                 hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+            float t = 0;
+            if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = std::min(ms, t);
+        }
+        ok = ok && pann_batch_download(b, nullptr, nullptr, nullptr, flags.data(), nullptr, nullptr, st) == PANN_OK &&
+             hipStreamSynchronize(st) == hipSuccess;
+        bool clean = ok;
+        for (uint32_t i = 0; clean && i < nq; i++) clean = (flags[i] & (kFlagOverflow | kFlagConfig)) == 0;
+        if (clean && ms < bestMs) {
+            bestMs = ms;
+            pann_batch_info(b, best, nullptr);
+            found = true;
+        }
+        pann_batch_free(b);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(st);
+    if (!found) return setError(PANN_ERR_CAPACITY, "no candidate configuration ran cleanly");
+    if (best_ms) *best_ms = bestMs;
+    return PANN_OK;
+}
+
+}  // extern "C"

@@ -97,7 +97,8 @@ struct FlatLaunch {
     uint32_t ldsPerWave;
     FlatLds lds;
     uint32_t* gfilt;           // per-wave global filter regions
-    uint32_t* queue;           // persistent mode: {next query, finished waves}; null: static
+    uint32_t* queue;           // persistent mode: {next ticket, finished waves}; null: static
+    const uint32_t* order;     // persistent mode: ticket -> query (null: identity)
     uint32_t totalWaves;
     // outputs
     uint32_t* outIds;
@@ -153,6 +154,7 @@ struct HnswLaunch {
     HnswLds lds;
     uint32_t* gfilt;
     uint32_t* queue;
+    const uint32_t* order;
     uint32_t totalWaves;
     uint32_t* outIds;
     int32_t* outDists;

@@ -43,6 +43,12 @@ class SearchOutputs(C.Structure):
                 ("stride", C.c_uint32), ("nq", C.c_uint32)]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("max_Ls", C.c_int32), ("max_k", C.c_int32), ("waves_per_block", C.c_int32),
+                ("lds_filter_bits", C.c_int32), ("visited_slots", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
 _u32 = C.c_uint32
@@ -90,6 +96,18 @@ PANN_SIGNATURES = {
     "pann_searcher_download": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "pann_searcher_free": (None, [_P]),
     "pann_bruteforce_knn": (C.c_int, [_P, _P, _u32, _u32, _P, _P, _P]),
+    "pann_batch_prepare_flat": (C.c_int, [_P, _P, _u32, C.POINTER(BatchConfig), _PP]),
+    "pann_batch_prepare_hnsw": (C.c_int, [_P, _u32, C.POINTER(BatchConfig), _PP]),
+    "pann_batch_set_queries": (C.c_int, [_P, _P, _P, _u32, _P]),
+    "pann_batch_launch": (C.c_int, [_P, _P]),
+    "pann_batch_capture": (C.c_int, [_P, _P]),
+    "pann_batch_replay": (C.c_int, [_P, _P]),
+    "pann_batch_outputs": (C.c_int, [_P, C.POINTER(SearchOutputs)]),
+    "pann_batch_download": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "pann_batch_info": (C.c_int, [_P, C.POINTER(BatchConfig), C.POINTER(_u32)]),
+    "pann_batch_free": (None, [_P]),
+    "pann_batch_autotune": (C.c_int, [_P, _P, _P, _P, _P, _u32, _i32, _i32,
+                                      C.POINTER(BatchConfig), C.POINTER(C.c_float)]),
 }
 
 TOOLS_SIGNATURES = {

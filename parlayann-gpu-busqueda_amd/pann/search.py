@@ -285,6 +285,118 @@ def search_flat(base, graph, queries, cfg, qids=None):
         s.free()
 
 
+PARAMS_DTYPE = np.dtype([("qid", "<u4"), ("Ls", "<i4"), ("k", "<i4"), ("reserved", "<i4"),
+                         ("visit_limit", "<i8"), ("cut", "<f8")])
+
+
+def make_params(nq, Ls, k, visit_limit=None, cut=1.35, qids=None):
+    """pann_query_params records: scalars broadcast, arrays per query."""
+    p = np.zeros(nq, dtype=PARAMS_DTYPE)
+    p["qid"] = np.arange(nq) if qids is None else qids
+    p["Ls"] = Ls
+    p["k"] = k
+    p["visit_limit"] = 0 if visit_limit is None else visit_limit
+    p["cut"] = cut
+    return p
+
+
+class Batch:
+    """Batched persistent form: queries with their own qid/Ls/k/visit/cut drained by one
+    launch of resident wavefronts; optionally captured into a HIP graph and replayed."""
+
+    def __init__(self, max_queries, max_Ls, max_k, base=None, graph=None, hnsw=None,
+                 waves_per_block=0, lds_filter_bits=0, visited_slots=0, config=None):
+        cfg = config or _lib.BatchConfig(max_Ls, max_k, waves_per_block, lds_filter_bits,
+                                          visited_slots, 0)
+        h = C.c_void_p()
+        if hnsw is not None:
+            check(lib().pann_batch_prepare_hnsw(hnsw.handle, max_queries, C.byref(cfg), C.byref(h)))
+        else:
+            check(lib().pann_batch_prepare_flat(base.handle, graph.handle, max_queries,
+                                                C.byref(cfg), C.byref(h)))
+        self.handle = h.value
+        self.max_k = cfg.max_k
+        self.nq = 0
+        self._keep = None
+
+    def set_queries(self, rows, params, stream=None):
+        rows = np.ascontiguousarray(rows, dtype=np.int8)
+        params = np.ascontiguousarray(params, dtype=PARAMS_DTYPE)
+        self._keep = (rows, params)  # alive until the asynchronous copies complete
+        check(lib().pann_batch_set_queries(self.handle, _ptr(rows), _ptr(params), len(params), stream))
+        self.nq = len(params)
+
+    def launch(self, stream=None):
+        check(lib().pann_batch_launch(self.handle, stream))
+
+    def capture(self, stream):
+        check(lib().pann_batch_capture(self.handle, stream))
+
+    def replay(self, stream=None):
+        check(lib().pann_batch_replay(self.handle, stream))
+
+    def config(self):
+        c, w = _lib.BatchConfig(), C.c_uint32()
+        check(lib().pann_batch_info(self.handle, C.byref(c), C.byref(w)))
+        return {f: getattr(c, f) for f, _ in c._fields_}, w.value
+
+    def download(self, stream=None):
+        nq, k = self.nq, self.max_k
+        r = Results(np.empty((nq, k), np.uint32), np.empty((nq, k), np.int32),
+                    np.empty(nq, np.int32), np.empty(nq, np.int32), np.empty(nq, np.int32),
+                    np.empty(nq, np.int32))
+        check(lib().pann_batch_download(self.handle, _ptr(r.ids), _ptr(r.dists), _ptr(r.comps),
+                                        _ptr(r.flags), _ptr(r.edges), _ptr(r.hops), stream))
+        if stream is not None:
+            synchronize(stream)
+        return r
+
+    def free(self):
+        if self.handle:
+            lib().pann_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def autotune(rows, params, max_Ls, max_k, base=None, graph=None, hnsw=None):
+    """Fastest clean batch configuration on this sample: (BatchConfig, ms per launch)."""
+    rows = np.ascontiguousarray(rows, dtype=np.int8)
+    params = np.ascontiguousarray(params, dtype=PARAMS_DTYPE)
+    best, ms = _lib.BatchConfig(), C.c_float()
+    check(lib().pann_batch_autotune(base.handle if base is not None else None,
+                                    graph.handle if graph is not None else None,
+                                    hnsw.handle if hnsw is not None else None, _ptr(rows),
+                                    _ptr(params), len(params), max_Ls, max_k, C.byref(best),
+                                    C.byref(ms)))
+    return best, ms.value
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        check(lib().pann_stream_create(C.byref(s)))
+        self.ptr = s.value
+
+    def synchronize(self):
+        synchronize(self.ptr)
+
+    def free(self):
+        if self.ptr:
+            lib().pann_stream_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def bruteforce_knn(base, queries, k, nq=None, stream=None):
     """Exact top-k on the GPU (MFMA distances, fused selection): ids u32, dists i32 [nq, k]."""
     nq = queries.n if nq is None else nq

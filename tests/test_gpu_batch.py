@@ -1,0 +1,143 @@
+"""Batched persistent form: every query's result equals the plain one-launch-per-config
+form (and the oracle), whatever mix of Ls / k / visit limits shares the launch; graph
+replays match the eager launch; the autotuner returns a configuration that runs cleanly."""
+import numpy as np
+import pytest
+
+from hnsw_util import random_hnsw  # noqa: F401  (shared helpers live beside the tests)
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_params(nq, n_base, rng, ls_lo=10, ls_hi=200, ks=(1, 10, 100)):
+    from pann import search
+    Ls = np.rint(np.exp(rng.uniform(np.log(ls_lo), np.log(ls_hi), size=nq))).astype(np.int32)
+    k = rng.choice(ks, size=nq).astype(np.int32)
+    qids = rng.integers(0, n_base, size=nq).astype(np.uint32)
+    return search.make_params(nq, Ls, k, visit_limit=2 * Ls, qids=qids)
+
+
+def _oracle_params(params):
+    from oracle import oracle as O
+    return O.make_params(len(params), params["Ls"], params["k"], params["visit_limit"],
+                         params["cut"], params["qid"])
+
+
+def _check_against_oracle(r, o, params, tag):
+    for i in range(len(params)):
+        k = int(params["k"][i])
+        same = (np.array_equal(r.ids[i, :k], o.ids[i, :k]) and np.array_equal(r.dists[i, :k], o.dists[i, :k])
+                and r.comps[i] == o.comps[i] and r.flags[i] == o.flags[i] and r.hops[i] == o.hops[i])
+        assert same, (f"{tag}: query {i} (Ls={params['Ls'][i]}, k={k}) gpu {r.ids[i, :k]} c{r.comps[i]} "
+                      f"f{r.flags[i]} | oracle {o.ids[i, :k]} c{o.comps[i]} f{o.flags[i]}")
+
+
+def test_flat_batch_matches_oracle_and_plain(gpu, small):
+    from oracle import oracle as O
+    from pann import search
+    base, queries, graph = small
+    rng = np.random.default_rng(1)
+    params = _mixed_params(len(queries), len(base), rng)
+    b, g = search.Vectors(base), search.FlatGraph(graph)
+    batch = search.Batch(len(queries), 200, 100, base=b, graph=g)
+    batch.set_queries(queries, params)
+    batch.launch()
+    search.synchronize()
+    r = batch.download()
+    o = O.beam_flat(base, graph, queries, _oracle_params(params))
+    _check_against_oracle(r, o, params, "flat batch")
+    # the same query alone, through the plain searcher, gives the same answer
+    q1 = search.Vectors(queries[:1])
+    for i in (0, 7, 33):
+        q1.copy_async(queries[i:i + 1])
+        search.synchronize()
+        p = params[i]
+        qid = search.DeviceBuffer.from_array(np.array([p["qid"]], np.uint32))
+        s = search.Searcher(g, 1, search.SearchConfig(Ls=int(p["Ls"]), k=int(p["k"]),
+                                                      visit_limit=int(p["visit_limit"])))
+        s.launch(b, q1, g, qid)
+        search.synchronize()
+        a = s.download()
+        assert np.array_equal(a.ids[0], r.ids[i, :p["k"]]) and a.comps[0] == r.comps[i]
+        s.free(), qid.free()
+    batch.free(), q1.free(), g.free(), b.free()
+
+
+def test_hnsw_batch_matches_oracle(gpu):
+    from oracle import oracle as O
+    from pann import builders, datagen, search
+    base, queries = datagen.gmm_dataset(4000, 300, dim=100, ncomp=12, seeds=(71, 72, 73))
+    h = builders.build_hnsw(base, M=16, efc=64, seed=2)
+    rng = np.random.default_rng(2)
+    params = _mixed_params(len(queries), len(base), rng, ls_hi=300)
+    b = search.Vectors(base)
+    idx = search.HnswIndex(h, b)
+    batch = search.Batch(len(queries), 300, 100, hnsw=idx)
+    batch.set_queries(queries, params)
+    batch.launch()
+    search.synchronize()
+    r = batch.download()
+    o = O.search_hnsw(base, h, queries, _oracle_params(params))
+    _check_against_oracle(r, o, params, "hnsw batch")
+    batch.free(), idx.free(), b.free()
+
+
+def test_graph_replay_matches_eager(gpu, small):
+    """Captured launch replayed several times, with eager launches and synchronisations in
+    between; every replay equals the eager result."""
+    from pann import search
+    base, queries, graph = small
+    params = _mixed_params(len(queries), len(base), np.random.default_rng(3), ls_hi=120)
+    b, g = search.Vectors(base), search.FlatGraph(graph)
+    st = search.Stream()
+    batch = search.Batch(len(queries), 120, 100, base=b, graph=g)
+    batch.set_queries(queries, params, st.ptr)
+    batch.launch(st.ptr)
+    st.synchronize()
+    eager = batch.download()
+    batch.capture(st.ptr)
+    for rep in range(4):
+        batch.replay(st.ptr)
+        st.synchronize()
+        r = batch.download()
+        assert np.array_equal(r.ids, eager.ids) and np.array_equal(r.dists, eager.dists), rep
+        assert np.array_equal(r.comps, eager.comps)
+        batch.launch(st.ptr)
+        st.synchronize()
+    batch.free(), st.free(), g.free(), b.free()
+
+
+@pytest.mark.parametrize("wpb,bits,slots", [(1, 10, 1024), (2, 11, 2048), (4, 12, 4096)])
+def test_batch_shapes_agree(gpu, small, wpb, bits, slots):
+    from pann import search
+    base, queries, graph = small
+    params = _mixed_params(len(queries), len(base), np.random.default_rng(4), ls_hi=150)
+    b, g = search.Vectors(base), search.FlatGraph(graph)
+    ref = search.Batch(len(queries), 150, 100, base=b, graph=g)
+    ref.set_queries(queries, params)
+    ref.launch()
+    search.synchronize()
+    want = ref.download()
+    other = search.Batch(len(queries), 150, 100, base=b, graph=g, waves_per_block=wpb,
+                         lds_filter_bits=bits, visited_slots=slots)
+    other.set_queries(queries, params)
+    other.launch()
+    search.synchronize()
+    got = other.download()
+    assert np.array_equal(got.ids, want.ids) and np.array_equal(got.comps, want.comps)
+    ref.free(), other.free(), g.free(), b.free()
+
+
+def test_autotune_returns_clean_config(gpu, small):
+    from pann import search
+    base, queries, graph = small
+    params = _mixed_params(len(queries), len(base), np.random.default_rng(5), ls_hi=100)
+    b, g = search.Vectors(base), search.FlatGraph(graph)
+    cfg, ms = search.autotune(queries, params, 100, 100, base=b, graph=g)
+    assert ms > 0 and cfg.waves_per_block in (1, 2, 4) and cfg.max_Ls == 100
+    batch = search.Batch(len(queries), 100, 100, base=b, graph=g, config=cfg)
+    batch.set_queries(queries, params)
+    batch.launch()
+    search.synchronize()
+    assert (batch.download().flags & 5 == 0).all()
+    batch.free(), g.free(), b.free()
