@@ -124,7 +124,11 @@ def test_batch_shapes_agree(gpu, small, wpb, bits, slots):
     other.launch()
     search.synchronize()
     got = other.download()
-    assert np.array_equal(got.ids, want.ids) and np.array_equal(got.comps, want.comps)
+    # only the first k columns of a query are written
+    for i, k in enumerate(params["k"]):
+        assert np.array_equal(got.ids[i, :k], want.ids[i, :k]), i
+        assert np.array_equal(got.dists[i, :k], want.dists[i, :k]), i
+    assert np.array_equal(got.comps, want.comps) and np.array_equal(got.flags, want.flags)
     ref.free(), other.free(), g.free(), b.free()
 
 
