@@ -107,7 +107,8 @@ def test_graph_replay_matches_eager(gpu, small):
     batch.free(), st.free(), g.free(), b.free()
 
 
-@pytest.mark.parametrize("wpb,bits,slots", [(1, 10, 1024), (2, 11, 2048), (4, 12, 4096)])
+@pytest.mark.parametrize("wpb,bits,slots", [(1, 10, 1024), (2, 11, 2048), (1, 12, 4096),
+                                             (4, 10, 512)])
 def test_batch_shapes_agree(gpu, small, wpb, bits, slots):
     from pann import search
     base, queries, graph = small
