@@ -116,6 +116,145 @@ def cpu_baseline(base, graph, queries, Ls, k, seconds):
                       f"ParlayANN beam_search"}
 
 
+def homogeneous_batched(b, g, queries, Ls, k, steps, warmup):
+    """The same uniform workload through the batched persistent form (graph-replayed):
+    returns (wall ms per launch, event ms per launch, results)."""
+    from pann import _lib, search
+    import ctypes as C
+    L = _lib.lib()
+    nq = len(queries)
+    params = search.make_params(nq, Ls, k)
+    st = search.Stream()
+    batch = search.Batch(nq, Ls, k, base=b, graph=g)
+    batch.set_queries(queries, params, st.ptr)
+    batch.launch(st.ptr)
+    batch.capture(st.ptr)
+    for _ in range(warmup):
+        batch.replay(st.ptr)
+    st.synchronize()
+    ev = []
+    for _ in range(steps + 1):
+        e = C.c_void_p()
+        _lib.check(L.pann_event_create(C.byref(e)))
+        ev.append(e.value)
+    _lib.check(L.pann_device_synchronize())
+    t0 = time.perf_counter()
+    _lib.check(L.pann_event_record(ev[0], st.ptr))
+    for i in range(steps):
+        batch.replay(st.ptr)
+        _lib.check(L.pann_event_record(ev[i + 1], st.ptr))
+    st.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    ms, tot = C.c_float(), 0.0
+    for i in range(steps):
+        _lib.check(L.pann_event_elapsed_ms(ev[i], ev[i + 1], C.byref(ms)))
+        tot += ms.value
+    r = batch.download(st.ptr)
+    for e in ev:
+        L.pann_event_destroy(e)
+    cfg, waves = batch.config()
+    batch.free()
+    st.free()
+    return wall, tot / steps, r, cfg, waves
+
+
+def mixed_workload(queries, nq_total, seed=4):
+    """65,536 queries resampled cyclically from the query set (each keeps its own qid):
+    Ls = round(exp(U[ln 10, ln 400])), k in {1, 10, 100}, visit limit 2*Ls."""
+    from pann import search
+    rng = np.random.Generator(np.random.PCG64(seed))
+    idx = np.arange(nq_total) % len(queries)
+    Ls = np.rint(np.exp(rng.uniform(np.log(10), np.log(400), size=nq_total))).astype(np.int32)
+    k = rng.choice(np.array([1, 10, 100], np.int32), size=nq_total)
+    params = search.make_params(nq_total, Ls, k, visit_limit=2 * Ls, qids=idx.astype(np.uint32))
+    return queries[idx], params
+
+
+def batched_vs_plain(b, g, rows, params, steps, tune_sample=8192):
+    """QPS of one autotuned persistent launch (HIP-graph replayed) against the plain form:
+    one prepared searcher per distinct (Ls, k, visit), launched one after another as the
+    reference's per-configuration protocol does.  Also checks every query agrees."""
+    from pann import _lib, search
+    import ctypes as C
+    L = _lib.lib()
+    nq = len(params)
+    max_Ls, max_k = int(params["Ls"].max()), int(params["k"].max())
+    t = time.time()
+    cfg, tune_ms = search.autotune(rows[:tune_sample], params[:tune_sample], max_Ls, max_k,
+                                   base=b, graph=g)
+    log(f"[bench] autotune {time.time() - t:.1f}s -> waves/block {cfg.waves_per_block}, LDS filter "
+        f"bits {cfg.lds_filter_bits}, visited slots {cfg.visited_slots}")
+    st = search.Stream()
+    batch = search.Batch(nq, max_Ls, max_k, base=b, graph=g, config=cfg)
+    batch.set_queries(rows, params, st.ptr)
+    batch.launch(st.ptr)
+    batch.capture(st.ptr)
+    st.synchronize()
+    ev = []
+    for _ in range(2):
+        e = C.c_void_p()
+        _lib.check(L.pann_event_create(C.byref(e)))
+        ev.append(e.value)
+    _lib.check(L.pann_device_synchronize())
+    t0 = time.perf_counter()
+    _lib.check(L.pann_event_record(ev[0], st.ptr))
+    for _ in range(steps):
+        batch.replay(st.ptr)
+    _lib.check(L.pann_event_record(ev[1], st.ptr))
+    st.synchronize()
+    batched_ms = (time.perf_counter() - t0) * 1e3 / steps
+    ms = C.c_float()
+    _lib.check(L.pann_event_elapsed_ms(ev[0], ev[1], C.byref(ms)))
+    batched_kernel_ms = ms.value / steps
+    rb = batch.download(st.ptr)
+
+    # plain form: prepared outside the timed region
+    keys = np.stack([params["Ls"], params["k"], params["visit_limit"]], axis=1)
+    groups = {}
+    for i, key in enumerate(map(tuple, keys)):
+        groups.setdefault(key, []).append(i)
+    plain = []
+    for (Ls, k, visit), members in groups.items():
+        members = np.array(members)
+        qv = search.Vectors(rows[members])
+        qid = search.DeviceBuffer.from_array(params["qid"][members].astype(np.uint32))
+        s = search.Searcher(g, len(members), search.SearchConfig(Ls=int(Ls), k=int(k), visit_limit=int(visit)))
+        plain.append((members, qv, qid, s))
+    _lib.check(L.pann_device_synchronize())
+    for members, qv, qid, s in plain:  # warm-up
+        s.launch(b, qv, g, qid, st.ptr)
+    st.synchronize()
+    t0 = time.perf_counter()
+    for members, qv, qid, s in plain:
+        s.launch(b, qv, g, qid, st.ptr)
+    st.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    mismatches = 0
+    for members, qv, qid, s in plain:
+        r = s.download()
+        k = s.cfg.k
+        mismatches += int(np.count_nonzero(
+            (r.ids[:, :k] != rb.ids[members, :k]).any(axis=1) | (r.comps != rb.comps[members])))
+        s.free(), qid.free(), qv.free()
+    flags = int(np.count_nonzero(rb.flags & (search.FLAG_OVERFLOW | search.FLAG_CONFIG)))
+    batch.free()
+    st.free()
+    for e in ev:
+        L.pann_event_destroy(e)
+    return {
+        "workload": f"{nq} queries, Ls log-uniform [10,400], k in {{1,10,100}}, visit 2*Ls",
+        "distinct_configs": len(groups),
+        "qps_batched": round(nq / (batched_ms / 1e3), 1),
+        "qps_plain": round(nq / (plain_ms / 1e3), 1),
+        "ratio": round(plain_ms / batched_ms, 2),
+        "batched_ms": round(batched_ms, 3), "batched_kernel_ms": round(batched_kernel_ms, 3),
+        "plain_ms": round(plain_ms, 3),
+        "autotuned": {"waves_per_block": cfg.waves_per_block, "lds_filter_bits": cfg.lds_filter_bits,
+                      "visited_slots": cfg.visited_slots, "tune_ms": round(tune_ms, 3)},
+        "queries_differing_from_plain": mismatches, "overflow_or_config_flags": flags,
+    }
+
+
 def calibrate_copy(nbytes, reps):
     """Runs the plain copy kernel `reps` times over `nbytes` (byte-counter calibration)."""
     from pann import _lib, search
@@ -143,6 +282,8 @@ def main():
     ap.add_argument("--cache", default=os.environ.get("PANN_BENCH_CACHE", ""))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the batched-vs-plain workload")
+    ap.add_argument("--mixed-nq", type=int, default=65_536)
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run the copy kernel over this many bytes (counter calibration)")
@@ -192,7 +333,15 @@ def main():
     if chosen is None:
         raise SystemExit("target recall not reached by the sweep")
     Ls, rec, s, r = chosen
-    wall_ms, ev_ms = time_launches(s, b, q, g, args.steps, args.warmup)
+    plain_wall, plain_ev = time_launches(s, b, q, g, args.steps, args.warmup)
+    log(f"[bench] plain Ls={Ls}: {plain_wall:.3f} ms/launch wall, {plain_ev:.3f} ms kernel, "
+        f"{args.nq / plain_wall * 1e3:,.0f} QPS")
+    bat_wall, bat_ev, rbat, bcfg, bwaves = homogeneous_batched(b, g, queries, Ls, k, args.steps, args.warmup)
+    same = bool(np.array_equal(rbat.ids[:, :k], r.ids) and np.array_equal(rbat.comps, r.comps))
+    log(f"[bench] batched Ls={Ls}: {bat_wall:.3f} ms/launch wall, {bat_ev:.3f} ms kernel, "
+        f"{args.nq / bat_wall * 1e3:,.0f} QPS, identical to plain: {same}")
+    form = "batched persistent (HIP graph replay)" if (bat_wall < plain_wall and same) else "plain"
+    wall_ms, ev_ms = (bat_wall, bat_ev) if form != "plain" else (plain_wall, plain_ev)
     qps = args.nq / (wall_ms / 1e3)
     alg_bytes = float(100 * r.comps.astype(np.float64).sum() + 4 * r.edges.astype(np.float64).sum())
     achieved = alg_bytes / (ev_ms / 1e3) / 1e9
@@ -206,6 +355,14 @@ def main():
             traffic = None
     log(f"[bench] Ls={Ls}: {wall_ms:.3f} ms/launch wall, {ev_ms:.3f} ms kernel, {qps:,.0f} QPS, "
         f"{achieved:.1f} GB/s algorithmic")
+
+    mixed = None
+    if not args.no_mixed:
+        rows, params = mixed_workload(queries, args.mixed_nq)
+        mixed = batched_vs_plain(b, g, rows, params, max(3, args.steps // 2))
+        log(f"[bench] mixed: batched {mixed['qps_batched']:,.0f} QPS vs plain {mixed['qps_plain']:,.0f} "
+            f"QPS ({mixed['ratio']}x over {mixed['distinct_configs']} configs), "
+            f"{mixed['queries_differing_from_plain']} queries differ")
 
     cpu = None
     if not args.no_cpu:
@@ -226,9 +383,18 @@ def main():
         "dtype": "int8 (exact int32 distances)",
         "data": "synthetic: Gaussian mixture, int8",
         "config": {
-            "workload": (f"plain launch, {args.n}x100 int8 GMM({args.ncomp}), {args.nq} queries, "
-                         f"Vamana R={args.R} L={args.L} alpha={args.alpha}, k=10, cut=1.35"),
-            "graph": "vamana", "n": args.n, "nq": args.nq, "Ls": Ls, "recall@10": round(rec, 5),
+            "workload": (f"{args.n}x100 int8 GMM({args.ncomp}), {args.nq} queries, "
+                         f"Vamana R={args.R} L={args.L} alpha={args.alpha}, k=10, cut=1.35, "
+                         f"smallest Ls with recall@10 >= {args.target}; one step = one {form} "
+                         f"launch over all queries"),
+            "form": form, "graph": "vamana", "n": args.n, "nq": args.nq, "Ls": Ls,
+            "recall@10": round(rec, 5),
+            "plain_qps": round(args.nq / plain_wall * 1e3, 1), "plain_kernel_ms": round(plain_ev, 4),
+            "batched_qps": round(args.nq / bat_wall * 1e3, 1), "batched_kernel_ms": round(bat_ev, 4),
+            "batched_identical_to_plain": same,
+            "batched_shape": {"waves_per_block": bcfg["waves_per_block"],
+                              "lds_filter_bits": bcfg["lds_filter_bits"],
+                              "visited_slots": bcfg["visited_slots"], "resident_waves": bwaves},
             "kernel_ms": round(ev_ms, 4), "comps_per_query": round(float(r.comps.mean()), 1),
             "edges_per_query": round(float(r.edges.mean()), 1),
             "hops_per_query": round(float(r.hops.mean()), 1),
@@ -238,6 +404,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
         "cpu_baseline": cpu,
+        "batched_vs_plain": mixed,
     }
     s.free(), g.free(), q.free(), b.free()
     print(json.dumps(out), flush=True)
