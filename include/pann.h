@@ -202,6 +202,57 @@ int pann_batch_autotune(const pann_vectors* base, const pann_graph* g, const pan
                         const int8_t* host_rows, const pann_query_params* params, uint32_t nq,
                         int32_t max_Ls, int32_t max_k, pann_batch_config* best, float* best_ms);
 
+/* ---- serving loop (no counterpart in the reference) ----
+ * A dispatcher thread keeps the index resident and packs the queries of arriving
+ * requests into micro-batches of the batched form.  A request is admitted as soon as
+ * device query slots (the HBM budget) are free; up to `streams` micro-batches run at
+ * once on their own streams, so requests that arrive while others are on the GPU start
+ * without waiting for them to finish.  Latency is measured from submission to the moment
+ * the request's results are in host memory. */
+typedef struct pann_server pann_server;
+
+typedef struct pann_server_config {
+    uint32_t slots;           /* queries resident on the device at once (HBM budget) */
+    uint32_t max_batch;       /* queries per micro-batch launch */
+    int32_t streams;          /* micro-batches in flight at once */
+    int32_t max_Ls;           /* per-query limits (LDS sizing) */
+    int32_t max_k;
+    int32_t waves_per_block;  /* batch shape, 0 = defaults (see pann_batch_config) */
+    int32_t lds_filter_bits;
+    uint32_t visited_slots;
+} pann_server_config;
+
+typedef struct pann_server_stats {
+    uint64_t requests;        /* completed */
+    uint64_t queries;
+    uint64_t launches;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* request latency */
+    double busy_s;            /* first submission to last completion */
+} pann_server_stats;
+
+int pann_server_create(const pann_vectors* base, const pann_graph* g, const pann_hnsw* h,
+                       const pann_server_config* cfg, pann_server** out);
+/* Copies the request's rows (dim bytes each) and parameters; thread-safe. */
+int pann_server_submit(pann_server* s, const int8_t* rows, const pann_query_params* params,
+                       uint32_t nq, uint64_t* request_id);
+/* Blocks until the request completes, copies nq x max_k ids/dists and nq flags (any may be
+ * NULL) and its latency; the request is then forgotten. */
+int pann_server_wait(pann_server* s, uint64_t request_id, uint32_t* ids, int32_t* dists,
+                     int32_t* flags, double* latency_ms);
+int pann_server_stats_get(pann_server* s, pann_server_stats* out);
+void pann_server_free(pann_server* s);
+
+/* Open-loop load generator: Poisson arrivals of `rate_qps` queries per second for
+ * duration_s, request sizes log-uniform in [1, max_request], per-query Ls log-uniform in
+ * [Ls_lo, Ls_hi], k drawn from ks[nks], qid = row index into rows[nrows].  Fills *stats;
+ * when record_ids is not NULL, the first min(k, 10) ids of up to record_cap served queries
+ * are written there (10 per query) with their row indices in record_rows. */
+int pann_server_run_poisson(pann_server* s, const int8_t* rows, uint32_t nrows, double rate_qps,
+                            double duration_s, uint32_t max_request, int32_t Ls_lo, int32_t Ls_hi,
+                            const int32_t* ks, uint32_t nks, uint64_t seed,
+                            pann_server_stats* stats, uint32_t* record_ids, uint32_t* record_rows,
+                            uint32_t record_cap, uint32_t* recorded);
+
 /* ---- exact ground truth ---- */
 /* Exact top-k (k <= 128) of the first nq queries over all base rows, ordered by
  * (distance, id), written to device arrays ids/dists (nq x k).  int8 -> int32 MFMA

@@ -262,15 +262,21 @@ int pann_batch_set_queries(pann_batch* b, const int8_t* host_rows, const pann_qu
 int pann_batch_launch(pann_batch* b, void* stream) {
     if (!b) return setError(PANN_ERR_ARG, "batch is NULL");
     if (b->nq == 0) return PANN_OK;
+    // a small batch launches only as many wavefronts as it has queries
+    const int blocks = (int)std::min<uint32_t>((uint32_t)b->blocks,
+                                               (b->nq + b->wavesPerBlock - 1) / b->wavesPerBlock);
+    const uint32_t waves = (uint32_t)blocks * b->wavesPerBlock;
     hipError_t e;
     if (b->kind == 0) {
         FlatLaunch L = b->L;
         L.nq = b->nq;
-        e = launchBeamFlat(L, b->maxch, b->wavesPerBlock, b->blocks, (hipStream_t)stream);
+        L.totalWaves = waves;
+        e = launchBeamFlat(L, b->maxch, b->wavesPerBlock, blocks, (hipStream_t)stream);
     } else {
         HnswLaunch H = b->H;
         H.nq = b->nq;
-        e = launchBeamHnsw(H, b->wr, b->maxch, b->wavesPerBlock, b->blocks, (hipStream_t)stream);
+        H.totalWaves = waves;
+        e = launchBeamHnsw(H, b->wr, b->maxch, b->wavesPerBlock, blocks, (hipStream_t)stream);
     }
     if (e != hipSuccess) return setError(PANN_ERR_HIP, "batched launch: %s", hipGetErrorString(e));
     return PANN_OK;

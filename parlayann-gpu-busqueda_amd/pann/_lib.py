@@ -49,6 +49,18 @@ class BatchConfig(C.Structure):
                 ("reserved", C.c_uint32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("max_batch", C.c_uint32), ("streams", C.c_int32),
+                ("max_Ls", C.c_int32), ("max_k", C.c_int32), ("waves_per_block", C.c_int32),
+                ("lds_filter_bits", C.c_int32), ("visited_slots", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("queries", C.c_uint64), ("launches", C.c_uint64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("max_ms", C.c_double), ("busy_s", C.c_double)]
+
+
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
 _u32 = C.c_uint32
@@ -108,6 +120,14 @@ PANN_SIGNATURES = {
     "pann_batch_free": (None, [_P]),
     "pann_batch_autotune": (C.c_int, [_P, _P, _P, _P, _P, _u32, _i32, _i32,
                                       C.POINTER(BatchConfig), C.POINTER(C.c_float)]),
+    "pann_server_create": (C.c_int, [_P, _P, _P, C.POINTER(ServerConfig), _PP]),
+    "pann_server_submit": (C.c_int, [_P, _P, _P, _u32, C.POINTER(_u64)]),
+    "pann_server_wait": (C.c_int, [_P, _u64, _P, _P, _P, C.POINTER(C.c_double)]),
+    "pann_server_stats_get": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "pann_server_free": (None, [_P]),
+    "pann_server_run_poisson": (C.c_int, [_P, _P, _u32, C.c_double, C.c_double, _u32, _i32, _i32,
+                                          _P, _u32, _u64, C.POINTER(ServerStats), _P, _P, _u32,
+                                          C.POINTER(_u32)]),
 }
 
 TOOLS_SIGNATURES = {

@@ -376,6 +376,72 @@ def autotune(rows, params, max_Ls, max_k, base=None, graph=None, hnsw=None):
     return best, ms.value
 
 
+class Server:
+    """Continuous-batching serving loop over a resident index (pann_server_*)."""
+
+    def __init__(self, slots, max_batch, streams, max_Ls, max_k, base=None, graph=None, hnsw=None,
+                 waves_per_block=0, lds_filter_bits=0, visited_slots=0):
+        cfg = _lib.ServerConfig(slots, max_batch, streams, max_Ls, max_k, waves_per_block,
+                                lds_filter_bits, visited_slots)
+        h = C.c_void_p()
+        check(lib().pann_server_create(base.handle if base is not None else None,
+                                       graph.handle if graph is not None else None,
+                                       hnsw.handle if hnsw is not None else None, C.byref(cfg),
+                                       C.byref(h)))
+        self.handle = h.value
+        self.max_k = max_k
+
+    def submit(self, rows, params):
+        rows = np.ascontiguousarray(rows, dtype=np.int8)
+        params = np.ascontiguousarray(params, dtype=PARAMS_DTYPE)
+        rid = C.c_uint64()
+        check(lib().pann_server_submit(self.handle, _ptr(rows), _ptr(params), len(params), C.byref(rid)))
+        return rid.value, len(params)
+
+    def wait(self, ticket):
+        rid, nq = ticket
+        ids = np.empty((nq, self.max_k), np.uint32)
+        d = np.empty((nq, self.max_k), np.int32)
+        f = np.empty(nq, np.int32)
+        ms = C.c_double()
+        check(lib().pann_server_wait(self.handle, rid, _ptr(ids), _ptr(d), _ptr(f), C.byref(ms)))
+        return ids, d, f, ms.value
+
+    def stats(self):
+        s = _lib.ServerStats()
+        check(lib().pann_server_stats_get(self.handle, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def run_poisson(self, rows, rate_qps, duration_s, max_request=4096, Ls_lo=10, Ls_hi=300,
+                    ks=(10, 100), seed=5, record_cap=0):
+        """Open-loop Poisson load; returns (stats dict, recorded ids [m, 10], recorded rows)."""
+        rows = np.ascontiguousarray(rows, dtype=np.int8)
+        ks = np.ascontiguousarray(ks, dtype=np.int32)
+        s = _lib.ServerStats()
+        rec_ids = np.empty((max(1, record_cap), 10), np.uint32)
+        rec_rows = np.empty(max(1, record_cap), np.uint32)
+        got = C.c_uint32()
+        check(lib().pann_server_run_poisson(self.handle, _ptr(rows), len(rows), float(rate_qps),
+                                            float(duration_s), max_request, Ls_lo, Ls_hi, _ptr(ks),
+                                            len(ks), seed, C.byref(s),
+                                            _ptr(rec_ids) if record_cap else None,
+                                            _ptr(rec_rows) if record_cap else None, record_cap,
+                                            C.byref(got)))
+        st = {f: getattr(s, f) for f, _ in s._fields_}
+        return st, rec_ids[:got.value], rec_rows[:got.value]
+
+    def free(self):
+        if self.handle:
+            lib().pann_server_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Stream:
     def __init__(self):
         s = C.c_void_p()
