@@ -255,6 +255,28 @@ def batched_vs_plain(b, g, rows, params, steps, tune_sample=8192):
     }
 
 
+def serving(b, g, queries, gt_ids, gt_d, rates, duration):
+    """Poisson arrivals on the resident index: requests of 1..4096 queries, per-query Ls
+    log-uniform in [10, 300], k in {10, 100}; p50/p99 request latency (arrival to results
+    in host memory), sustained QPS and recall@10 of the served queries."""
+    from pann import recall, search
+    srv = search.Server(slots=65536, max_batch=16384, streams=3, max_Ls=300, max_k=100, base=b, graph=g)
+    out = []
+    srv.run_poisson(queries, 50000, 0.3, seed=99)  # warm-up
+    for i, rate in enumerate(rates):
+        st, ids, rows = srv.run_poisson(queries, rate, duration, max_request=4096, Ls_lo=10, Ls_hi=300,
+                                        ks=(10, 100), seed=7 + i, record_cap=200000)
+        rec = recall.recall_at_k(gt_ids[rows], gt_d[rows], ids, 10) if len(rows) else None
+        out.append({"offered_qps": rate, "sustained_qps": round(st["queries"] / max(st["busy_s"], 1e-9), 1),
+                    "requests": int(st["requests"]), "queries": int(st["queries"]),
+                    "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                    "max_ms": round(st["max_ms"], 3), "recall@10": round(rec, 5) if rec is not None else None})
+        log(f"[bench] serving {rate:,.0f} QPS offered: sustained {out[-1]['sustained_qps']:,.0f}, "
+            f"p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms, recall@10 {out[-1]['recall@10']}")
+    srv.free()
+    return out
+
+
 def calibrate_copy(nbytes, reps):
     """Runs the plain copy kernel `reps` times over `nbytes` (byte-counter calibration)."""
     from pann import _lib, search
@@ -284,6 +306,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the batched-vs-plain workload")
     ap.add_argument("--mixed-nq", type=int, default=65_536)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-rates", default="200000,1000000,2000000")
+    ap.add_argument("--serving-seconds", type=float, default=2.0)
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run the copy kernel over this many bytes (counter calibration)")
@@ -364,6 +389,11 @@ def main():
             f"QPS ({mixed['ratio']}x over {mixed['distinct_configs']} configs), "
             f"{mixed['queries_differing_from_plain']} queries differ")
 
+    serve = None
+    if not args.no_serving:
+        serve = serving(b, g, queries, gt_ids, gt_df, [float(x) for x in args.serving_rates.split(",")],
+                        args.serving_seconds)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(base, graph, queries, Ls, k, args.cpu_seconds)
@@ -405,6 +435,7 @@ def main():
                      "traffic": traffic},
         "cpu_baseline": cpu,
         "batched_vs_plain": mixed,
+        "serving": serve,
     }
     s.free(), g.free(), q.free(), b.free()
     print(json.dumps(out), flush=True)
