@@ -99,6 +99,11 @@ void pann_event_destroy(void* ev);
 /* Plain streaming copy kernel (16-byte vector loads/stores, bytes % 16 == 0): a launch of
  * known traffic used to scale hardware byte counters. */
 int pann_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
+/* Per-section cycle sums of the flat search, as the reference's -DPERFIL build keeps
+ * them (cuda/busqueda.cu:15-32): init, visit, filter, distances, sort, union, merge,
+ * cut, next, output, then the hop and query counts.  Only a library built with
+ * `build.py --profile` (libpann_prof.so) collects them; others return PANN_ERR_CAPACITY. */
+int pann_profile_counters(uint64_t* out, int n, int reset);
 
 /* ---- vectors (VectoresGPU) ---- */
 /* subirVectores (cuda/datos_gpu.cu:20-40): host rows of `dim` int8 -> device rows with a

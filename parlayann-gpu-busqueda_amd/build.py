@@ -44,15 +44,27 @@ def run(cmd):
     return r
 
 
-def compile_one(src, obj, hip):
+def compile_one(src, obj, hip, extra=()):
     if hip:
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj]
+        cmd = [HIPCC] + HIP_FLAGS + list(extra) + ["-c", src, "-o", obj]
         if src.endswith(".cpp"):
-            cmd = [HIPCC] + HIP_FLAGS + ["-x", "hip", "-c", src, "-o", obj]
+            cmd = [HIPCC] + HIP_FLAGS + list(extra) + ["-x", "hip", "-c", src, "-o", obj]
     else:
         cmd = [CXX] + HOST_FLAGS + ["-c", src, "-o", obj]
     run(cmd)
     return obj
+
+
+def build_profile(jobs=8):
+    """libpann_prof.so: the same library with per-section cycle counters compiled in."""
+    os.makedirs(OBJ, exist_ok=True)
+    tasks = [(os.path.join(CSRC, s), os.path.join(OBJ, "prof_" + s + ".o"), True, ("-DPANN_PROFILE",))
+             for s in PANN_HIP_SOURCES]
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda t: compile_one(*t), tasks))
+    out = os.path.join(HERE, "libpann_prof.so")
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out + ".tmp"] + objs)
+    os.replace(out + ".tmp", out)
 
 
 def build(jobs=8):
@@ -84,8 +96,11 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
     ap.add_argument("--clean", action="store_true")
+    ap.add_argument("--profile", action="store_true", help="also build libpann_prof.so")
     a = ap.parse_args()
     if a.clean:
         clean()
     build(a.jobs)
+    if a.profile:
+        build_profile(a.jobs)
     print("built", os.path.join(HERE, "libpann.so"))

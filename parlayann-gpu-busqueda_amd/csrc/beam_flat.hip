@@ -33,7 +33,7 @@ struct QueryCfg {
 
 template <int MAXCH, bool FGLOBAL>
 __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, const QueryCfg& c,
-                          uint32_t* gfilt) {
+                          uint32_t* gfilt, Prof& P) {
     const int lane = laneId();
     const int grp = lane >> 3, sub = lane & 7;
     const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -89,6 +89,8 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
     uint32_t distinctVisited = 0;
     const uint32_t G = L.G;
     const uint32_t nch = (G + 63) / 64;
+    P.mark(kProfInit);
+    P.count(kProfQueries);
 
     while (u >= 0 && visits < c.limit) {
         const uint64_t ukey = fkey[cur][u];
@@ -126,6 +128,8 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
         }
         visits++;
         hops++;
+        P.mark(kProfVisit);
+        P.count(kProfHops);
 
         const int cutoff = (fsize < Ls) ? 0x7FFFFFFF : (int)(fkey[cur][fsize - 1] >> 32);
 
@@ -144,6 +148,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
             __builtin_amdgcn_wave_barrier();
         }
         comps += (int)nP;
+        P.mark(kProfFilter);
 
         // ---- distances: 8 rows per round, 8 lanes x 16 B each ----
         uint64_t key[MAXCH];
@@ -175,8 +180,10 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
             }
         }
 
+        P.mark(kProfDist);
         // ---- sort candidates by (d, id) ----
         bitonicSort<MAXCH>(key, lane);
+        P.mark(kProfSort);
 
         // ---- multiset union with the frontier ----
         const uint64_t* F = fkey[cur];
@@ -244,6 +251,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
         const int nK = (int)kBase;
         __builtin_amdgcn_wave_barrier();
 
+        P.mark(kProfUnion);
         // ---- merged frontier: each position finds its source by rank ----
         const int nxt = cur ^ 1;
         const int newSize = min(Ls, fsize + nK);
@@ -276,6 +284,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
         cur = nxt;
         fsize = newSize;
 
+        P.mark(kProfMerge);
         // ---- cut: upper_bound(F, (0, (float)(cut * (double)F[k].d))) ----
         if (c.k > 0 && fsize > c.k) {
             const int dk = (int)(fkey[cur][c.k] >> 32);
@@ -298,6 +307,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
             fsize = nf;
         }
 
+        P.mark(kProfCut);
         // ---- next: first frontier copy not visited ----
         u = -1;
         for (int p0 = 0; p0 < fsize; p0 += 64) {
@@ -309,6 +319,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
                 break;
             }
         }
+        P.mark(kProfNext);
     }
 
     // ---- output: first k of the frontier (cuda/busqueda.cu:487-497) ----
@@ -332,6 +343,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
         if (L.outHops) L.outHops[q] = hops;
     }
     __builtin_amdgcn_wave_barrier();
+    P.mark(kProfOutput);
 }
 
 template <int MAXCH>
@@ -344,6 +356,8 @@ __global__ __launch_bounds__(256) void beamFlatKernel(FlatLaunch L) {
     unsigned char* lds = smem + (size_t)waveInBlock * L.ldsPerWave;
     uint32_t* gfilt = L.gfilt ? L.gfilt + ((size_t)waveGlobal << L.gbitsMax) : nullptr;
 
+    Prof P;
+    P.start();
     uint32_t q = L.queue ? takeQuery(L.queue, L.order, L.nq, lane) : waveGlobal;
     while (q < L.nq) {
         QueryCfg c;
@@ -366,16 +380,36 @@ __global__ __launch_bounds__(256) void beamFlatKernel(FlatLaunch L) {
             (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
             if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
         } else if (fb <= L.fbitsLds) {
-            searchOne<MAXCH, false>(L, lds, q, c, nullptr);
+            searchOne<MAXCH, false>(L, lds, q, c, nullptr, P);
         } else {
-            searchOne<MAXCH, true>(L, lds, q, c, gfilt);
+            searchOne<MAXCH, true>(L, lds, q, c, gfilt, P);
         }
         q = L.queue ? takeQuery(L.queue, L.order, L.nq, lane) : q + L.totalWaves;
     }
+    P.flush(lane);
     if (L.queue) finishQueue(L.queue, L.totalWaves, lane);
 }
 
 }  // namespace
+
+#ifdef PANN_PROFILE
+namespace dev {
+__device__ unsigned long long g_profile[kProfSections];
+}
+
+hipError_t profileCounters(uint64_t* out, int n, bool reset) {
+    unsigned long long h[kProfSections] = {};
+    hipError_t e = hipMemcpyFromSymbol(h, HIP_SYMBOL(dev::g_profile), sizeof(h));
+    for (int i = 0; i < n && i < kProfSections; i++) out[i] = h[i];
+    if (e == hipSuccess && reset) {
+        unsigned long long z[kProfSections] = {};
+        e = hipMemcpyToSymbol(HIP_SYMBOL(dev::g_profile), z, sizeof(z));
+    }
+    return e;
+}
+#else
+hipError_t profileCounters(uint64_t*, int, bool) { return hipErrorNotSupported; }
+#endif
 
 hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int blocks,
                           hipStream_t stream) {

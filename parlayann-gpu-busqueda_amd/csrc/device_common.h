@@ -6,6 +6,42 @@
 namespace pann {
 namespace dev {
 
+// ---- per-section cycle accounting (compiled in only with -DPANN_PROFILE) ----
+// Sections of the flat search; each wavefront sums clock64() deltas and adds them to
+// g_profile once when it exits (the reference's -DPERFIL idea, cuda/busqueda.cu:15-32).
+enum ProfSection {
+    kProfInit = 0, kProfVisit, kProfFilter, kProfDist, kProfSort, kProfUnion, kProfMerge,
+    kProfCut, kProfNext, kProfOutput, kProfHops, kProfQueries, kProfSections
+};
+#ifdef PANN_PROFILE
+extern __device__ unsigned long long g_profile[kProfSections];
+struct Prof {
+    unsigned long long acc[kProfSections];
+    long long t;
+    __device__ void start() {
+        for (int i = 0; i < kProfSections; i++) acc[i] = 0;
+        t = clock64();
+    }
+    __device__ void mark(int s) {
+        const long long n = clock64();
+        acc[s] += (unsigned long long)(n - t);
+        t = n;
+    }
+    __device__ void count(int s) { acc[s]++; }
+    __device__ void flush(int lane) {
+        if (lane == 0)
+            for (int i = 0; i < kProfSections; i++) atomicAdd(&g_profile[i], acc[i]);
+    }
+};
+#else
+struct Prof {
+    __device__ void start() {}
+    __device__ void mark(int) {}
+    __device__ void count(int) {}
+    __device__ void flush(int) {}
+};
+#endif
+
 // parlaylib hash64_2 (cuda/utiles_gpu.cuh:12-17): slot of ParlayANN's seen filter.
 __device__ __forceinline__ uint64_t hash64_2(uint64_t x) {
     x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;

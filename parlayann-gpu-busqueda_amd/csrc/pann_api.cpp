@@ -172,6 +172,14 @@ void pann_event_destroy(void* ev) {
     if (ev) hipEventDestroy((hipEvent_t)ev);
 }
 
+int pann_profile_counters(uint64_t* out, int n, int reset) {
+    if (!out || n < 0) return setError(PANN_ERR_ARG, "NULL argument");
+    const hipError_t e = profileCounters(out, n, reset != 0);
+    if (e == hipErrorNotSupported) return setError(PANN_ERR_CAPACITY, "not a profiling build");
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "profile counters: %s", hipGetErrorString(e));
+    return PANN_OK;
+}
+
 int pann_copy_kernel(void* dst, const void* src, size_t bytes, void* stream) {
     if (!dst || !src || bytes % 16) return setError(PANN_ERR_ARG, "copy needs 16-byte multiples");
     HIP_TRY(launchCopy(dst, src, bytes, (hipStream_t)stream));

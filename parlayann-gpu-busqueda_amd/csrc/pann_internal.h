@@ -174,6 +174,8 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
                           hipStream_t stream);
 hipError_t beamFlatOccupancy(int maxch, int wavesPerBlock, size_t ldsBytes, int* blocksPerCU);
 hipError_t launchRowNorms(const int8_t* rows, uint32_t n, int32_t* out, hipStream_t s);
+// Per-section cycle counters of the flat search (profiling builds only).
+hipError_t profileCounters(uint64_t* out, int n, bool reset);
 hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_t n,
                                const int8_t* queries, const int32_t* qnorm, uint32_t nq,

@@ -5,7 +5,9 @@ import ctypes as C
 import os
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_ROOT, "libpann.so")
+# PANN_PROFILE=1 selects the section-profiling build (build.py --profile)
+LIB_PATH = os.path.join(PKG_ROOT, "libpann_prof.so" if os.environ.get("PANN_PROFILE") == "1"
+                        else "libpann.so")
 TOOLS_PATH = os.path.join(PKG_ROOT, "libpann_tools.so")
 
 PANN_OK = 0
@@ -86,6 +88,7 @@ PANN_SIGNATURES = {
     "pann_event_elapsed_ms": (C.c_int, [_P, _P, C.POINTER(C.c_float)]),
     "pann_event_destroy": (None, [_P]),
     "pann_copy_kernel": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "pann_profile_counters": (C.c_int, [_P, C.c_int, C.c_int]),
     "pann_vectors_create": (C.c_int, [_P, _u32, _u32, _PP]),
     "pann_vectors_load": (C.c_int, [C.c_char_p, _PP]),
     "pann_vectors_copy_async": (C.c_int, [_P, _P, _u32, _P]),
