@@ -176,7 +176,7 @@ typedef struct pann_batch_config {
     int32_t max_k;           /* output stride */
     int32_t waves_per_block; /* 1, 2 or 4 (0: the most that fit 64 KB of LDS) */
     int32_t lds_filter_bits; /* seen filters up to 2^bits slots live in LDS, 10..12 (0: 12) */
-    uint32_t visited_slots;  /* LDS visited / is_inw hash slots, power of two (0: derived) */
+    uint32_t visited_slots;  /* LDS visited / is_inw hash slots, power of two >= 64 (0: derived) */
     uint32_t reserved;
 } pann_batch_config;
 

@@ -346,7 +346,323 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
     P.mark(kProfOutput);
 }
 
-template <int MAXCH>
+// ---------------------------------------------------------------------------------------
+// Register-resident form for Ls <= 64 and rows of at most 64 entries.  Same semantics as
+// searchOne; lane i holds frontier entry i as one tagged 64-bit key
+//     d (24 bits) | id (32 bits) | tag (8 bits)
+// (an exact int8 squared distance is below 128 * 255^2 < 2^23).  Tag bit 7 is 0 for a
+// frontier copy, whose bits 1..6 hold its position (equal keys keep their order through
+// the merge) and bit 0 its visited flag; it is 1 for a candidate, whose bits 0..6 hold
+// how often its id was expanded (saturated at 127; a row has at most 64 copies).
+//   * candidate vectors are loaded straight from the row layout (entry e = lane e) as
+//     soon as the filter slots are read, while in-row slot collisions are resolved;
+//   * the candidates are sorted descending with DPP / permlane exchanges and merged
+//     with the frontier by one bitonic merge.  In a run of equal keys the cF frontier
+//     copies come first; candidate copy rep (0-based) enters iff rep >= cF, visited iff
+//     rep < expansions(id) -- the multiset union of the reference's search.
+constexpr uint64_t kNoKey = ~0ull;
+
+__device__ __forceinline__ uint64_t tagKey(int d, uint32_t id, uint32_t tag) {
+    return ((uint64_t)(uint32_t)d << 40) | ((uint64_t)id << 8) | tag;
+}
+__device__ __forceinline__ int keyDist(uint64_t k) { return (int)(k >> 40); }
+__device__ __forceinline__ uint32_t keyId(uint64_t k) { return (uint32_t)(k >> 8); }
+
+// Value of lane 8 * (lane / 8) + r (ds_swizzle bit mode: and 0x18, or r).
+__device__ __forceinline__ uint32_t groupBroadcast8(uint32_t x, int r) {
+    switch (r) {
+        case 0: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (0 << 5));
+        case 1: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (1 << 5));
+        case 2: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (2 << 5));
+        case 3: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (3 << 5));
+        case 4: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (4 << 5));
+        case 5: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (5 << 5));
+        case 6: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (6 << 5));
+        default: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (7 << 5));
+    }
+}
+
+template <bool DESC>
+__device__ __forceinline__ uint64_t sortWave(uint64_t x, int lane) {
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const bool asc = ((lane & k) == 0) != DESC;
+            x = cmpXchg(x, j, ((lane & j) == 0) == asc, lane);
+        }
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint64_t mergeHalf(uint64_t x, int lane) {
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1) x = cmpXchg(x, j, (lane & j) == 0, lane);
+    return x;
+}
+
+// Highest set bit of m at or below this lane (upTo = lanes 0..lane), or -1.
+__device__ __forceinline__ int lastAtOrBelow(uint64_t m, uint64_t upTo) {
+    const uint64_t x = m & upTo;
+    return x ? 63 - __clzll((long long)x) : -1;
+}
+
+struct UnionMarks {
+    uint64_t drop;  // candidate copies the union drops
+    int lastRun;    // run start / first candidate copy seen by lane 63 (carried to the
+    int lastC;      // next 64 positions of the merged sequence)
+};
+
+// Multiset-union marks over positions base..base+63 of the merged sequence; x holds this
+// lane's entry, prev the entry before it (kNoKey before position 0).  Candidate copies
+// get their visited flag in bit 0 (bits 1..6 cleared).
+__device__ __forceinline__ UnionMarks unionMarks(uint64_t& x, uint64_t prev, int base,
+                                                 int carryRun, int carryC, int lane,
+                                                 uint64_t upTo) {
+    const bool valid = x != kNoKey;
+    const bool isC = valid && (x & 0x80u);
+    const bool same = valid && ((x ^ prev) >> 8) == 0;
+    const bool prevC = (prev & 0x80u) != 0;
+    int run = lastAtOrBelow(ballot(valid && !same), upTo);
+    run = run < 0 ? carryRun : base + run;
+    int fc = lastAtOrBelow(ballot(isC && !(same && prevC)), upTo);
+    fc = fc < 0 ? carryC : base + fc;
+    bool drop = false;
+    if (isC) {
+        const int p = base + lane;
+        drop = p < 2 * fc - run;  // rep = p - fc < cF = fc - run
+        const uint32_t vis = (p - fc) < (int)(x & 0x7Fu) ? 1u : 0u;
+        x = (x & ~0x7Full) | vis;
+    }
+    UnionMarks m;
+    m.drop = ballot(drop);
+    m.lastRun = readlane(run, 63);
+    m.lastC = readlane(fc, 63);
+    return m;
+}
+
+__device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, const QueryCfg& c,
+                           Prof& P) {
+    const int lane = laneId();
+    const int grp = lane >> 3, sub = lane & 7;
+    const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    uint64_t* scratch = reinterpret_cast<uint64_t*>(lds + L.lds.kkey);
+    unsigned char* tag = lds + L.lds.tag;
+    uint32_t* vkey = reinterpret_cast<uint32_t*>(lds + L.lds.vkey);
+    unsigned char* vcnt = lds + L.lds.vcnt;
+    uint32_t* filt = reinterpret_cast<uint32_t*>(lds + L.lds.filt);
+    const uint64_t upTo = lanesBelow | (1ull << lane);
+
+    const int Ls = c.Ls;
+    const uint32_t fmask = (1u << kFastFilterBits) - 1;
+    uint32_t capV = flatVisitedCap(Ls);
+    if (capV > L.capV) capV = L.capV;
+    const uint32_t logCap = (uint32_t)ceilLog2(capV);
+    const uint32_t vmask = capV - 1;
+    const uint32_t win = capV < 64 ? capV : 64;
+
+    const int4 qv = *reinterpret_cast<const int4*>(L.queries + (size_t)q * kVecBytes + sub * 16);
+    const int qq = groupSum8(sliceNorm(qv));
+
+    {
+        const uint4 e4 = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+        filtClear<false>(filt, kFastFilterBits, lane);
+        uint4* v4 = reinterpret_cast<uint4*>(vkey);
+        for (uint32_t i = lane; i < capV / 4; i += 64) v4[i] = e4;
+    }
+
+    // start point: node 0 (cuda/busqueda.cu:166-173)
+    const int d0 = groupDistance(L.base, 0u, sub, qv, qq);
+    uint64_t F = lane == 0 ? tagKey(d0, 0u, 0u) : kNoKey;
+    int fsize = 1;
+    int u = 0;
+    int64_t visits = 0;
+    int comps = 1, edges = 0, hops = 0, flags = 0;
+    uint32_t distinct = 0;
+    const uint32_t G = L.G;
+    __builtin_amdgcn_wave_barrier();
+    P.mark(kProfInit);
+    P.count(kProfQueries);
+
+    while (u >= 0 && visits < c.limit) {
+        const uint32_t uid = keyId(readlane64(F, u));
+        const uint32_t a = (uint32_t)lane < G ? L.adj[(size_t)uid * G + lane] : kEmpty;
+
+        // visit u: its flag, and its expansion count (64 probe slots read at once)
+        if (lane == u) F |= 1u;
+        bool overflow = false;
+        {
+            const uint32_t h0 = visitedSlot(uid, logCap);
+            for (uint32_t off = 0; off < capV; off += win) {
+                const uint32_t s = (h0 + off + (uint32_t)lane) & vmask;
+                const bool act = (uint32_t)lane < win;
+                const uint32_t kk = act ? vkey[s] : 0u;
+                const uint64_t hit = ballot(act && kk == uid);
+                const uint64_t m = hit | ballot(act && kk == kEmpty);
+                if (m) {
+                    const int p = __ffsll((unsigned long long)m) - 1;
+                    if ((hit >> p) & 1ull) {
+                        if (lane == p) vcnt[s] = (unsigned char)(vcnt[s] + 1);
+                    } else if (distinct + 1 < capV) {
+                        if (lane == p) {
+                            vkey[s] = uid;
+                            vcnt[s] = 1;
+                        }
+                        distinct++;
+                    } else {
+                        overflow = true;
+                    }
+                    break;
+                }
+            }
+        }
+        if (overflow) {
+            flags |= kFlagOverflow;
+            break;
+        }
+        visits++;
+        hops++;
+        P.mark(kProfVisit);
+        P.count(kProfHops);
+
+        const int cutoff = (fsize < Ls) ? 0x7FFFFFFF
+                                        : (readlane((int)(uint32_t)(F >> 32), fsize - 1) >> 8);
+
+        // seen filter, first half: slots and tags
+        edges += __popcll(ballot(a != kEmpty));
+        const bool part = (a != kEmpty) && (a != c.qid);
+        const SeenProbe sp = seenFilterBegin<false>(filt, tag, fmask, a, part, lane);
+        const bool spec = part && (sp.coll || sp.prev != a);
+
+        // vectors of every entry that may pass: group g loads entry 8g + r in round r
+        const uint32_t gmask = (uint32_t)(ballot(spec) >> (8 * grp)) & 0xFFu;
+        uint32_t ids[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) ids[r] = groupBroadcast8(a, r);
+        int4 v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            v[r] = make_int4(0, 0, 0, 0);
+            if ((gmask >> r) & 1u)
+                v[r] = *reinterpret_cast<const int4*>(L.base + (size_t)ids[r] * kVecBytes + sub * 16);
+        }
+
+        // expansions of the id so far (decide the visited flags of entering copies)
+        uint32_t cnt = 0;
+        if (spec) {
+            uint32_t hh = visitedSlot(a, logCap);
+            while (true) {
+                const uint32_t kk = vkey[hh];
+                if (kk == a) {
+                    cnt = min(127u, (uint32_t)vcnt[hh]);
+                    break;
+                }
+                if (kk == kEmpty) break;
+                hh = (hh + 1) & vmask;
+            }
+        }
+
+        // seen filter, second half: in-row collisions, slot writes
+        const bool keep = seenFilterEnd<false>(filt, sp, a, part, lane, lanesBelow);
+        const uint64_t keepM = ballot(keep);
+        comps += __popcll(keepM);
+        // A lane that won its byte tag but shares the slot with an earlier lane was not
+        // speculated when the slot held its id before the row; load those now.
+        const uint64_t late = keepM & ~ballot(spec);
+        if (late) {
+            const uint32_t lmask = (uint32_t)(late >> (8 * grp)) & 0xFFu;
+#pragma unroll
+            for (int r = 0; r < 8; r++)
+                if ((lmask >> r) & 1u)
+                    v[r] = *reinterpret_cast<const int4*>(L.base + (size_t)ids[r] * kVecBytes + sub * 16);
+        }
+        P.mark(kProfFilter);
+
+        int part8[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) part8[r] = slicePartial(v[r], qv);
+        const int d = qq + transposeReduce8(part8, lane);  // distance of entry `lane`
+        const uint64_t C = (keep && d < cutoff) ? tagKey(d, a, 0x80u | cnt) : kNoKey;
+        P.mark(kProfDist);
+
+        const uint64_t Cd = sortWave<true>(C, lane);
+        P.mark(kProfSort);
+
+        // bitonic merge of frontier (ascending) and candidates (descending)
+        uint64_t lo = F < Cd ? F : Cd;
+        lo = mergeHalf(lo, lane);
+        const UnionMarks ml = unionMarks(lo, prevLane64(lo, kNoKey, lane), 0, 0, 0, lane, upTo);
+        const int validLo = __popcll(ballot(lo != kNoKey));
+        int newSize;
+        if (ml.drop == 0) {
+            newSize = min(Ls, validLo);
+        } else {
+            uint64_t hi = F < Cd ? Cd : F;
+            hi = mergeHalf(hi, lane);
+            const UnionMarks mh = unionMarks(hi, prevLane64(hi, readlane64(lo, 63), lane), 64,
+                                             ml.lastRun, ml.lastC, lane, upTo);
+            const int validHi = __popcll(ballot(hi != kNoKey));
+            const int nDropLo = __popcll(ml.drop);
+            const int dLo = lane - (int)__popcll(ml.drop & lanesBelow);
+            const int dHi = 64 + lane - nDropLo - (int)__popcll(mh.drop & lanesBelow);
+            if (lo != kNoKey && !((ml.drop >> lane) & 1ull)) scratch[dLo] = lo;
+            if (hi != kNoKey && !((mh.drop >> lane) & 1ull) && dHi < 64) scratch[dHi] = hi;
+            __builtin_amdgcn_wave_barrier();
+            newSize = min(Ls, validLo + validHi - nDropLo - (int)__popcll(mh.drop));
+            lo = lane < newSize ? scratch[lane] : kNoKey;
+            __builtin_amdgcn_wave_barrier();
+        }
+        F = lane < newSize ? ((lo & ~0xFFull) | ((uint64_t)lane << 1) | (lo & 1u)) : kNoKey;
+        fsize = newSize;
+        P.mark(kProfMerge);
+
+        // cut: upper_bound(F, (0, (float)(cut * (double)F[k].d)))
+        if (c.k > 0 && fsize > c.k) {
+            const int dk = readlane((int)(uint32_t)(F >> 32), c.k) >> 8;
+            const float t = (float)(c.cut * (double)dk);
+            const float df = (float)keyDist(F);
+            const bool pred = lane < fsize && ((t < df) || (t == df && keyId(F) > 0u));
+            const uint64_t m = ballot(pred);
+            if (m) {
+                fsize = __ffsll((unsigned long long)m) - 1;
+                if (lane >= fsize) F = kNoKey;
+            }
+        }
+        P.mark(kProfCut);
+
+        const uint64_t nv = ballot(lane < fsize && !(F & 1u));
+        u = nv ? __ffsll((unsigned long long)nv) - 1 : -1;
+        P.mark(kProfNext);
+    }
+
+    // output: first k of the frontier (cuda/busqueda.cu:487-497)
+    if (fsize < c.k) flags |= kFlagShort;
+    uint32_t* oi = L.outIds + (size_t)q * L.outStride;
+    int32_t* od = L.outDists + (size_t)q * L.outStride;
+    for (int j = lane; j < c.k; j += 64) {
+        if (j < fsize) {
+            oi[j] = keyId(F);
+            od[j] = keyDist(F);
+        } else {
+            oi[j] = kEmpty;
+            od[j] = 0x7FFFFFFF;
+        }
+    }
+    if (lane == 0) {
+        if (L.outComps) L.outComps[q] = comps;
+        if (L.outFlags) L.outFlags[q] = flags;
+        if (L.outEdges) L.outEdges[q] = edges;
+        if (L.outHops) L.outHops[q] = hops;
+    }
+    __builtin_amdgcn_wave_barrier();
+    P.mark(kProfOutput);
+}
+
+// FAST: every query of the launch takes searchFast (Ls <= 64, rows <= 64, layout
+// flatFastLdsLayout); otherwise queries with Ls <= 64 on rows of at most 64 entries take
+// it when the launch holds a filter of at least kFastFilterBits in LDS.
+template <int MAXCH, bool FAST>
 __global__ __launch_bounds__(256) void beamFlatKernel(FlatLaunch L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = laneId();
@@ -376,13 +692,23 @@ __global__ __launch_bounds__(256) void beamFlatKernel(FlatLaunch L) {
             c.cut = L.cut;
         }
         const int fb = flatFilterBits(c.Ls);
-        if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride ||
-            (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
-            if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
-        } else if (fb <= L.fbitsLds) {
-            searchOne<MAXCH, false>(L, lds, q, c, nullptr, P);
+        if constexpr (FAST) {
+            if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride) {
+                if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
+            } else {
+                searchFast(L, lds, q, c, P);
+            }
         } else {
-            searchOne<MAXCH, true>(L, lds, q, c, gfilt, P);
+            if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride ||
+                (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
+                if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
+            } else if (MAXCH == 1 && c.Ls <= kFastLsMax && L.fbitsLds >= kFastFilterBits) {
+                searchFast(L, lds, q, c, P);
+            } else if (fb <= L.fbitsLds) {
+                searchOne<MAXCH, false>(L, lds, q, c, nullptr, P);
+            } else {
+                searchOne<MAXCH, true>(L, lds, q, c, gfilt, P);
+            }
         }
         q = L.queue ? takeQuery(L.queue, L.order, L.nq, lane) : q + L.totalWaves;
     }
@@ -415,21 +741,32 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
                           hipStream_t stream) {
     const size_t lds = (size_t)L.ldsPerWave * wavesPerBlock;
     const dim3 grid(blocks), block(64 * wavesPerBlock);
+    if (L.fast) {
+        if (maxch != 1 || L.LsMax > kFastLsMax) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((beamFlatKernel<1, true>), grid, block, lds, stream, L);
+        return hipGetLastError();
+    }
     switch (maxch) {
-        case 1: hipLaunchKernelGGL(beamFlatKernel<1>, grid, block, lds, stream, L); break;
-        case 2: hipLaunchKernelGGL(beamFlatKernel<2>, grid, block, lds, stream, L); break;
-        case 4: hipLaunchKernelGGL(beamFlatKernel<4>, grid, block, lds, stream, L); break;
+        case 1: hipLaunchKernelGGL((beamFlatKernel<1, false>), grid, block, lds, stream, L); break;
+        case 2: hipLaunchKernelGGL((beamFlatKernel<2, false>), grid, block, lds, stream, L); break;
+        case 4: hipLaunchKernelGGL((beamFlatKernel<4, false>), grid, block, lds, stream, L); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t beamFlatOccupancy(int maxch, int wavesPerBlock, size_t ldsBytes, int* blocksPerCU) {
+hipError_t beamFlatOccupancy(int maxch, bool fast, int wavesPerBlock, size_t ldsBytes,
+                             int* blocksPerCU) {
     const int threads = 64 * wavesPerBlock;
+    if (fast) {
+        if (maxch != 1) return hipErrorInvalidValue;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, true>,
+                                                            threads, ldsBytes);
+    }
     switch (maxch) {
-        case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1>, threads, ldsBytes);
-        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<2>, threads, ldsBytes);
-        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<4>, threads, ldsBytes);
+        case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, false>, threads, ldsBytes);
+        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<2, false>, threads, ldsBytes);
+        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<4, false>, threads, ldsBytes);
         default: return hipErrorInvalidValue;
     }
 }

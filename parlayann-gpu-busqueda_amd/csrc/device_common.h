@@ -177,6 +177,50 @@ __device__ __forceinline__ int groupDistance(const int8_t* rows, uint32_t id, in
     return qq + groupSum8(slicePartial(v, qv));
 }
 
+// Value of lane (lane ^ j) with VALU cross-lane moves only (no LDS round trip):
+// j = 1, 2 quad permutes; 4 row shifts; 8 row rotate; 16 / 32 the gfx950 permlane swaps.
+__device__ __forceinline__ uint32_t xorLane32(uint32_t x, int j, int lane) {
+    switch (j) {
+        case 1: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);
+        case 2: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);
+        case 4: {
+            const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x104, 0xF, 0xF, false);
+            const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);
+            return (lane & 4) ? dn : up;
+        }
+        case 8: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x128, 0xF, 0xF, false);
+        case 16: {
+            const auto p = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+            return (lane & 16) ? p[0] : p[1];
+        }
+        default: {
+            const auto p = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+            return (lane & 32) ? p[0] : p[1];
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t xorLane64(uint64_t v, int j, int lane) {
+    const uint32_t lo = xorLane32((uint32_t)v, j, lane);
+    const uint32_t hi = xorLane32((uint32_t)(v >> 32), j, lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Value of lane - 1 (lane 0 gets `first`), a whole-wave DPP shift.
+__device__ __forceinline__ uint64_t prevLane64(uint64_t v, uint64_t first, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x138, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), 0x138, 0xF, 0xF, false);
+    return lane == 0 ? first : (((uint64_t)hi << 32) | lo);
+}
+
+// One compare-exchange stage at distance j < 64 inside a register.
+__device__ __forceinline__ uint64_t cmpXchg(uint64_t x, int j, bool keepMin, int lane) {
+    const uint64_t p = xorLane64(x, j, lane);
+    const uint64_t mn = x < p ? x : p;
+    const uint64_t mx = x < p ? p : x;
+    return keepMin ? mn : mx;
+}
+
 // Ascending bitonic sort of 64*C keys, element e = c*64 + lane in key[c].
 template <int C>
 __device__ __forceinline__ void bitonicSort(uint64_t (&key)[C], int lane) {
@@ -203,14 +247,25 @@ __device__ __forceinline__ void bitonicSort(uint64_t (&key)[C], int lane) {
                 for (int c = 0; c < C; c++) {
                     const int e = c * 64 + lane;
                     const bool asc = (e & k) == 0;
-                    const bool lower = (lane & j) == 0;
-                    const uint64_t p = shflXor64(key[c], j);
-                    const uint64_t mn = key[c] < p ? key[c] : p;
-                    const uint64_t mx = key[c] < p ? p : key[c];
-                    key[c] = (lower == asc) ? mn : mx;
+                    key[c] = cmpXchg(key[c], j, ((lane & j) == 0) == asc, lane);
                 }
             }
         }
+    }
+}
+
+// Turns two ascending-sorted registers' worth of a bitonic sequence (lo = elements 0..63,
+// hi = 64..127, lo ascending and hi descending) into one ascending run of 128.
+__device__ __forceinline__ void bitonicMerge128(uint64_t& lo, uint64_t& hi, int lane) {
+    const uint64_t mn = lo < hi ? lo : hi;
+    const uint64_t mx = lo < hi ? hi : lo;
+    lo = mn;
+    hi = mx;
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1) {
+        const bool keepMin = (lane & j) == 0;
+        lo = cmpXchg(lo, j, keepMin, lane);
+        hi = cmpXchg(hi, j, keepMin, lane);
     }
 }
 
@@ -251,21 +306,42 @@ __device__ __forceinline__ void filtClear(uint32_t* f, int bits, int lane) {
 // held a just before its turn, then the slot holds a.  Slot reads happen for the whole
 // chunk at once; lanes whose byte tag was overwritten share a slot (or a tag) with
 // another lane and are resolved by a scalar loop over those groups.  Returns keep.
+//
+// The work is split in two so that callers can issue loads between the halves:
+// seenFilterBegin reads slots and tags; seenFilterEnd resolves the lanes that share a
+// slot and writes the slots.  A lane alone in its slot is kept iff part && prev != a; a
+// lane sharing a slot may be kept although prev == a (an earlier lane of the row wrote
+// another id there first), and only the losers of the tag race see coll.
+struct SeenProbe {
+    uint32_t h;     // filter slot
+    uint32_t prev;  // slot value before this chunk
+    bool coll;      // shares a slot (or its byte tag) with another lane of the chunk
+};
+
 template <bool G>
-__device__ __forceinline__ bool seenFilterChunk(uint32_t* filt, unsigned char* tag, uint32_t fmask,
-                                                uint32_t a, bool part, int lane,
-                                                uint64_t lanesBelow) {
-    const uint32_t h = (uint32_t)hash64_2(a) & fmask;
-    const uint32_t ts = h & (kTagSlots - 1);
+__device__ __forceinline__ SeenProbe seenFilterBegin(const uint32_t* filt, unsigned char* tag,
+                                                     uint32_t fmask, uint32_t a, bool part,
+                                                     int lane) {
+    SeenProbe s;
+    s.h = (uint32_t)hash64_2(a) & fmask;
+    const uint32_t ts = s.h & (kTagSlots - 1);
     if (part) tag[ts] = (unsigned char)lane;
     __builtin_amdgcn_wave_barrier();
-    uint32_t prev = kEmpty;
-    bool coll = false;
+    s.prev = kEmpty;
+    s.coll = false;
     if (part) {
-        coll = tag[ts] != (unsigned char)lane;
-        prev = filtLoad<G>(filt, h);
+        s.coll = tag[ts] != (unsigned char)lane;
+        s.prev = filtLoad<G>(filt, s.h);
     }
-    uint64_t cm = ballot(coll);
+    return s;
+}
+
+template <bool G>
+__device__ __forceinline__ bool seenFilterEnd(uint32_t* filt, const SeenProbe& s, uint32_t a,
+                                              bool part, int lane, uint64_t lanesBelow) {
+    const uint32_t h = s.h;
+    uint32_t prev = s.prev;
+    uint64_t cm = ballot(s.coll);
     bool last = true;
     while (cm) {
         const int Lr = __ffsll((unsigned long long)cm) - 1;
@@ -285,6 +361,14 @@ __device__ __forceinline__ bool seenFilterChunk(uint32_t* filt, unsigned char* t
     if constexpr (G) __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
     return part && !seen;
+}
+
+template <bool G>
+__device__ __forceinline__ bool seenFilterChunk(uint32_t* filt, unsigned char* tag, uint32_t fmask,
+                                                uint32_t a, bool part, int lane,
+                                                uint64_t lanesBelow) {
+    const SeenProbe s = seenFilterBegin<G>(filt, tag, fmask, a, part, lane);
+    return seenFilterEnd<G>(filt, s, a, part, lane, lanesBelow);
 }
 
 // Persistent work queue: queue[0] is the next ticket, queue[1] counts finished waves.

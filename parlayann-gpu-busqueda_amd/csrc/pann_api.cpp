@@ -442,7 +442,8 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
     const int fbits = flatFilterBits(Ls);
     const bool ldsFilter = fbits <= kFilterBitsLds;
     const uint32_t capV = flatVisitedCap(Ls);
-    const FlatLds lay = flatLdsLayout(Ls, maxch, capV, ldsFilter ? fbits : 0);
+    const bool fast = maxch == 1 && Ls <= kFastLsMax;
+    const FlatLds lay = fast ? flatFastLdsLayout(capV) : flatLdsLayout(Ls, maxch, capV, ldsFilter ? fbits : 0);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "Ls=%d needs %u bytes of LDS per query", Ls, lay.total);
 
@@ -453,7 +454,7 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
     s->maxch = maxch;
     s->wavesPerBlock = (int)std::min<size_t>(4, std::max<size_t>(1, kMaxLdsPerBlock / lay.total));
     int bpc = 0, cus = 0;
-    hipError_t he = beamFlatOccupancy(maxch, s->wavesPerBlock, (size_t)lay.total * s->wavesPerBlock, &bpc);
+    hipError_t he = beamFlatOccupancy(maxch, fast, s->wavesPerBlock, (size_t)lay.total * s->wavesPerBlock, &bpc);
     r = deviceCUs(&cus);
     if (he != hipSuccess || r != PANN_OK) {
         delete s;
@@ -477,6 +478,7 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
     L.capV = capV;
     L.ldsPerWave = lay.total;
     L.lds = lay;
+    L.fast = fast ? 1 : 0;
     L.totalWaves = totalWaves;
     L.queue = nullptr;
     if (!ldsFilter) {

@@ -19,6 +19,7 @@ struct pann_batch {
     pann_batch_config cfg{};
     uint32_t maxQueries = 0, nq = 0, dim = 0;
     int maxch = 1, wr = 1, wavesPerBlock = 1, blocks = 0;
+    bool fast = false;  // flat: register-resident kernel
     uint32_t totalWaves = 0;
     FlatLaunch L{};
     HnswLaunch H{};
@@ -52,8 +53,9 @@ int checkBatchConfig(const pann_batch_config* cfg) {
         return setError(PANN_ERR_ARG, "waves_per_block must be 0, 1, 2 or 4");
     if (cfg->lds_filter_bits != 0 && (cfg->lds_filter_bits < 2 || cfg->lds_filter_bits > 12))
         return setError(PANN_ERR_ARG, "lds_filter_bits must be 0 or 2..12");
-    if (cfg->visited_slots && (cfg->visited_slots & (cfg->visited_slots - 1)))
-        return setError(PANN_ERR_ARG, "visited_slots must be a power of two");
+    if (cfg->visited_slots &&
+        ((cfg->visited_slots & (cfg->visited_slots - 1)) || cfg->visited_slots < 64))
+        return setError(PANN_ERR_ARG, "visited_slots must be 0 or a power of two >= 64");
     return PANN_OK;
 }
 
@@ -83,7 +85,7 @@ int finishSizing(pann_batch* b, uint32_t ldsPerWave, int gbits) {
     b->cfg.waves_per_block = wpb;
     int bpc = 0;
     hipError_t e = b->kind == 0
-                       ? beamFlatOccupancy(b->maxch, wpb, (size_t)ldsPerWave * wpb, &bpc)
+                       ? beamFlatOccupancy(b->maxch, b->fast, wpb, (size_t)ldsPerWave * wpb, &bpc)
                        : beamHnswOccupancy(b->wr, b->maxch, wpb, (size_t)ldsPerWave * wpb, &bpc);
     if (e != hipSuccess) return setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
     b->blocks = std::max(1, bpc) * cus;
@@ -118,7 +120,9 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
     const uint32_t capV = cfg->visited_slots ? cfg->visited_slots : flatVisitedCap(LsMax);
     b->cfg.lds_filter_bits = fbLds;
     b->cfg.visited_slots = capV;
-    const FlatLds lay = flatLdsLayout(LsMax, b->maxch, capV, fbLds);
+    const bool fast = b->maxch == 1 && LsMax <= kFastLsMax && fbLds >= kFastFilterBits;
+    b->fast = fast;
+    const FlatLds lay = fast ? flatFastLdsLayout(capV) : flatLdsLayout(LsMax, b->maxch, capV, fbLds);
     if (lay.total > kMaxLdsPerBlock) {
         delete b;
         return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsMax, lay.total);
@@ -138,6 +142,7 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
     L.capV = capV;
     L.ldsPerWave = lay.total;
     L.lds = lay;
+    L.fast = fast ? 1 : 0;
     L.gfilt = b->gfilt;
     L.queue = b->queue;
     L.order = b->order;

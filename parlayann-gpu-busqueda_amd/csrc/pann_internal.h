@@ -64,6 +64,24 @@ __host__ __device__ inline FlatLds flatLdsLayout(int LsMax, int maxch, uint32_t 
     return L;
 }
 
+// The register-resident form of the flat search (frontier of at most 64 entries, one
+// register per lane; adjacency rows of at most 64 entries) needs only the seen filter,
+// its byte tags, the expansion counts (visited hash) and a 64-entry compaction buffer.
+constexpr int kFastLsMax = 64;
+constexpr int kFastFilterBits = 10;  // flatFilterBits(Ls) for every Ls <= 64
+
+__host__ __device__ inline FlatLds flatFastLdsLayout(uint32_t capV) {
+    FlatLds L = {};
+    uint32_t o = 0;
+    L.kkey = o; o += 8u * kFastLsMax;
+    L.vkey = o; o += 4u * capV;
+    L.filt = o; o += 4u << kFastFilterBits;
+    L.tag = o;  o += kTagSlots;
+    L.vcnt = o; o += alignUp(capV, 8);
+    L.total = alignUp(o, 16);
+    return L;
+}
+
 // bits of ParlayANN's seen filter: max(10, ceil(log2(Ls*Ls)) - 2)   cuda/busqueda.cu:536-538
 __host__ __device__ inline int ceilLog2(uint64_t x) {
     int b = 0;
@@ -96,6 +114,7 @@ struct FlatLaunch {
     uint32_t capV;             // visited hash capacity in LDS (power of two)
     uint32_t ldsPerWave;
     FlatLds lds;
+    int32_t fast;              // 1: register-resident kernel, layout flatFastLdsLayout
     uint32_t* gfilt;           // per-wave global filter regions
     uint32_t* queue;           // persistent mode: {next ticket, finished waves}; null: static
     const uint32_t* order;     // persistent mode: ticket -> query (null: identity)
@@ -172,7 +191,8 @@ hipError_t beamHnswOccupancy(int wr, int maxch, int wavesPerBlock, size_t ldsByt
 // Launchers implemented in the .hip translation units.
 hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int blocks,
                           hipStream_t stream);
-hipError_t beamFlatOccupancy(int maxch, int wavesPerBlock, size_t ldsBytes, int* blocksPerCU);
+hipError_t beamFlatOccupancy(int maxch, bool fast, int wavesPerBlock, size_t ldsBytes,
+                             int* blocksPerCU);
 hipError_t launchRowNorms(const int8_t* rows, uint32_t n, int32_t* out, hipStream_t s);
 // Per-section cycle counters of the flat search (profiling builds only).
 hipError_t profileCounters(uint64_t* out, int n, bool reset);
