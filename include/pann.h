@@ -101,7 +101,9 @@ void pann_event_destroy(void* ev);
 int pann_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
 /* Per-section cycle sums of the flat search, as the reference's -DPERFIL build keeps
  * them (cuda/busqueda.cu:15-32): init, visit, filter, distances, sort, union, merge,
- * cut, next, output, then the hop and query counts.  Only a library built with
+ * cut, next, output, then the hop and query counts, then (register-resident form) the
+ * candidates below the cutoff summed over hops, the hops with none and the hops with
+ * 1..8 of them.  Only a library built with
  * `build.py --profile` (libpann_prof.so) collects them; others return PANN_ERR_CAPACITY. */
 int pann_profile_counters(uint64_t* out, int n, int reset);
 

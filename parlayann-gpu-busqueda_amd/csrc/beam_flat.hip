@@ -31,6 +31,21 @@ struct QueryCfg {
     double cut;
 };
 
+// The parameters are the same in every lane; saying so keeps them in scalar registers
+// and keeps branches on them uniform.
+__device__ __forceinline__ void makeUniform(QueryCfg& c) {
+    c.qid = (uint32_t)__builtin_amdgcn_readfirstlane((int)c.qid);
+    c.Ls = __builtin_amdgcn_readfirstlane(c.Ls);
+    c.k = __builtin_amdgcn_readfirstlane(c.k);
+    const uint64_t lim = (uint64_t)c.limit;
+    c.limit = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(lim >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)lim));
+    const uint64_t cb = (uint64_t)__double_as_longlong(c.cut);
+    c.cut = __longlong_as_double((long long)(
+        ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(cb >> 32)) << 32) |
+        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)cb)));
+}
+
 template <int MAXCH, bool FGLOBAL>
 __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, const QueryCfg& c,
                           uint32_t* gfilt, Prof& P) {
@@ -361,6 +376,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
 //     copies come first; candidate copy rep (0-based) enters iff rep >= cF, visited iff
 //     rep < expansions(id) -- the multiset union of the reference's search.
 constexpr uint64_t kNoKey = ~0ull;
+constexpr int kInsertMax = 16;  // up to this many candidates are inserted one by one
 
 __device__ __forceinline__ uint64_t tagKey(int d, uint32_t id, uint32_t tag) {
     return ((uint64_t)(uint32_t)d << 40) | ((uint64_t)id << 8) | tag;
@@ -585,40 +601,87 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
         const int d = qq + transposeReduce8(part8, lane);  // distance of entry `lane`
         const uint64_t C = (keep && d < cutoff) ? tagKey(d, a, 0x80u | cnt) : kNoKey;
         P.mark(kProfDist);
+        const uint64_t candM = ballot(C != kNoKey);
+        const int nc = __popcll(candM);
+        P.add(kProfCands, (unsigned long long)nc);
+        if (nc == 0) P.count(kProfNoCands);
+        if (nc > 0 && nc <= 8) P.count(kProfSmall);
 
-        const uint64_t Cd = sortWave<true>(C, lane);
-        P.mark(kProfSort);
+        bool changed = false;
+        if (nc > 0 && nc <= kInsertMax) {
+            // few candidates: insert each at its upper bound (equal keys: frontier copies,
+            // then the candidate copies already inserted), trimming to Ls as we go
+            int fs = fsize;
+            uint64_t mm = candM;
+            while (mm) {
+                const int j = __ffsll((unsigned long long)mm) - 1;
+                mm &= mm - 1;
+                const uint64_t x = readlane64(C, j);
+                const uint64_t xmin = x & ~0xFFull, xmax = x | 0xFFull;
+                const int pos = __popcll(ballot(F <= xmax));
+                if (pos >= Ls) continue;
+                const int lower = __popcll(ballot(F < xmin));
+                const int cnt = (int)(x & 0x7Fu);
+                uint32_t vis;
+                if (pos == lower) {
+                    // no copy of the key yet, so no earlier candidate copy either (it
+                    // would have been inserted, or dropped against a frontier copy)
+                    vis = cnt > 0 ? 1u : 0u;
+                } else {
+                    // rep counts the earlier candidate copies, inserted or dropped
+                    const int cF = __popcll(ballot(F >= xmin && F <= xmax && !(F & 0x80u)));
+                    const int rep = __popcll(ballot(C == x) & ((1ull << j) - 1));
+                    if (rep < cF) continue;
+                    vis = rep < cnt ? 1u : 0u;
+                }
+                const uint64_t xn = xmin | 0x80u | vis;
+                const uint64_t pv = prevLane64(F, kNoKey, lane);
+                F = lane < pos ? F : (lane == pos ? xn : pv);
+                fs = min(Ls, fs + 1);
+                changed = true;
+            }
+            if (changed) {
+                F = lane < fs ? ((F & ~0xFFull) | ((uint64_t)lane << 1) | (F & 1u)) : kNoKey;
+                fsize = fs;
+            }
+            P.mark(kProfMerge);
+        } else if (nc > kInsertMax) {
+            const uint64_t Cd = sortWave<true>(C, lane);
+            P.mark(kProfSort);
 
-        // bitonic merge of frontier (ascending) and candidates (descending)
-        uint64_t lo = F < Cd ? F : Cd;
-        lo = mergeHalf(lo, lane);
-        const UnionMarks ml = unionMarks(lo, prevLane64(lo, kNoKey, lane), 0, 0, 0, lane, upTo);
-        const int validLo = __popcll(ballot(lo != kNoKey));
-        int newSize;
-        if (ml.drop == 0) {
-            newSize = min(Ls, validLo);
-        } else {
-            uint64_t hi = F < Cd ? Cd : F;
-            hi = mergeHalf(hi, lane);
-            const UnionMarks mh = unionMarks(hi, prevLane64(hi, readlane64(lo, 63), lane), 64,
-                                             ml.lastRun, ml.lastC, lane, upTo);
-            const int validHi = __popcll(ballot(hi != kNoKey));
-            const int nDropLo = __popcll(ml.drop);
-            const int dLo = lane - (int)__popcll(ml.drop & lanesBelow);
-            const int dHi = 64 + lane - nDropLo - (int)__popcll(mh.drop & lanesBelow);
-            if (lo != kNoKey && !((ml.drop >> lane) & 1ull)) scratch[dLo] = lo;
-            if (hi != kNoKey && !((mh.drop >> lane) & 1ull) && dHi < 64) scratch[dHi] = hi;
-            __builtin_amdgcn_wave_barrier();
-            newSize = min(Ls, validLo + validHi - nDropLo - (int)__popcll(mh.drop));
-            lo = lane < newSize ? scratch[lane] : kNoKey;
-            __builtin_amdgcn_wave_barrier();
+            // bitonic merge of frontier (ascending) and candidates (descending)
+            uint64_t lo = F < Cd ? F : Cd;
+            lo = mergeHalf(lo, lane);
+            const UnionMarks ml = unionMarks(lo, prevLane64(lo, kNoKey, lane), 0, 0, 0, lane, upTo);
+            const int validLo = __popcll(ballot(lo != kNoKey));
+            int newSize;
+            if (ml.drop == 0) {
+                newSize = min(Ls, validLo);
+            } else {
+                uint64_t hi = F < Cd ? Cd : F;
+                hi = mergeHalf(hi, lane);
+                const UnionMarks mh = unionMarks(hi, prevLane64(hi, readlane64(lo, 63), lane), 64,
+                                                 ml.lastRun, ml.lastC, lane, upTo);
+                const int validHi = __popcll(ballot(hi != kNoKey));
+                const int nDropLo = __popcll(ml.drop);
+                const int dLo = lane - (int)__popcll(ml.drop & lanesBelow);
+                const int dHi = 64 + lane - nDropLo - (int)__popcll(mh.drop & lanesBelow);
+                if (lo != kNoKey && !((ml.drop >> lane) & 1ull)) scratch[dLo] = lo;
+                if (hi != kNoKey && !((mh.drop >> lane) & 1ull) && dHi < 64) scratch[dHi] = hi;
+                __builtin_amdgcn_wave_barrier();
+                newSize = min(Ls, validLo + validHi - nDropLo - (int)__popcll(mh.drop));
+                lo = lane < newSize ? scratch[lane] : kNoKey;
+                __builtin_amdgcn_wave_barrier();
+            }
+            F = lane < newSize ? ((lo & ~0xFFull) | ((uint64_t)lane << 1) | (lo & 1u)) : kNoKey;
+            fsize = newSize;
+            changed = true;
+            P.mark(kProfMerge);
         }
-        F = lane < newSize ? ((lo & ~0xFFull) | ((uint64_t)lane << 1) | (lo & 1u)) : kNoKey;
-        fsize = newSize;
-        P.mark(kProfMerge);
 
-        // cut: upper_bound(F, (0, (float)(cut * (double)F[k].d)))
-        if (c.k > 0 && fsize > c.k) {
+        // cut: upper_bound(F, (0, (float)(cut * (double)F[k].d))); a frontier that did
+        // not change since the last cut is already cut
+        if (changed && c.k > 0 && fsize > c.k) {
             const int dk = readlane((int)(uint32_t)(F >> 32), c.k) >> 8;
             const float t = (float)(c.cut * (double)dk);
             const float df = (float)keyDist(F);
@@ -691,6 +754,7 @@ __global__ __launch_bounds__(256) void beamFlatKernel(FlatLaunch L) {
             c.limit = L.limit;
             c.cut = L.cut;
         }
+        makeUniform(c);
         const int fb = flatFilterBits(c.Ls);
         if constexpr (FAST) {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride) {

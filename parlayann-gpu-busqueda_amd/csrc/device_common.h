@@ -11,7 +11,11 @@ namespace dev {
 // g_profile once when it exits (the reference's -DPERFIL idea, cuda/busqueda.cu:15-32).
 enum ProfSection {
     kProfInit = 0, kProfVisit, kProfFilter, kProfDist, kProfSort, kProfUnion, kProfMerge,
-    kProfCut, kProfNext, kProfOutput, kProfHops, kProfQueries, kProfSections
+    kProfCut, kProfNext, kProfOutput, kProfHops, kProfQueries,
+    kProfCands,    // candidates below the cutoff, summed over hops
+    kProfNoCands,  // hops with no candidate below the cutoff
+    kProfSmall,    // hops with 1..8 candidates below the cutoff
+    kProfSections
 };
 #ifdef PANN_PROFILE
 extern __device__ unsigned long long g_profile[kProfSections];
@@ -28,6 +32,7 @@ struct Prof {
         t = n;
     }
     __device__ void count(int s) { acc[s]++; }
+    __device__ void add(int s, unsigned long long v) { acc[s] += v; }
     __device__ void flush(int lane) {
         if (lane == 0)
             for (int i = 0; i < kProfSections; i++) atomicAdd(&g_profile[i], acc[i]);
@@ -38,6 +43,7 @@ struct Prof {
     __device__ void start() {}
     __device__ void mark(int) {}
     __device__ void count(int) {}
+    __device__ void add(int, unsigned long long) {}
     __device__ void flush(int) {}
 };
 #endif
@@ -177,18 +183,27 @@ __device__ __forceinline__ int groupDistance(const int8_t* rows, uint32_t id, in
     return qq + groupSum8(slicePartial(v, qv));
 }
 
+// DPP move with bound_ctrl set: lanes whose source falls outside the row read 0, so no
+// "old" value has to be materialised.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dppMov(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
+}
+
 // Value of lane (lane ^ j) with VALU cross-lane moves only (no LDS round trip):
 // j = 1, 2 quad permutes; 4 row shifts; 8 row rotate; 16 / 32 the gfx950 permlane swaps.
 __device__ __forceinline__ uint32_t xorLane32(uint32_t x, int j, int lane) {
     switch (j) {
-        case 1: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);
-        case 2: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);
+        case 1: return dppMov<0xB1>(x);
+        case 2: return dppMov<0x4E>(x);
         case 4: {
-            const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x104, 0xF, 0xF, false);
-            const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);
+            // both moves run with every lane active: a DPP read of a lane that is off
+            // in EXEC yields 0, so they must not sit inside the lane-dependent select
+            const uint32_t up = dppMov<0x104>(x);
+            const uint32_t dn = dppMov<0x114>(x);
             return (lane & 4) ? dn : up;
         }
-        case 8: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x128, 0xF, 0xF, false);
+        case 8: return dppMov<0x128>(x);
         case 16: {
             const auto p = __builtin_amdgcn_permlane16_swap(x, x, false, false);
             return (lane & 16) ? p[0] : p[1];
@@ -208,17 +223,16 @@ __device__ __forceinline__ uint64_t xorLane64(uint64_t v, int j, int lane) {
 
 // Value of lane - 1 (lane 0 gets `first`), a whole-wave DPP shift.
 __device__ __forceinline__ uint64_t prevLane64(uint64_t v, uint64_t first, int lane) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x138, 0xF, 0xF, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), 0x138, 0xF, 0xF, false);
+    const uint32_t lo = dppMov<0x138>((uint32_t)v);
+    const uint32_t hi = dppMov<0x138>((uint32_t)(v >> 32));
     return lane == 0 ? first : (((uint64_t)hi << 32) | lo);
 }
 
-// One compare-exchange stage at distance j < 64 inside a register.
+// One compare-exchange stage at distance j < 64 inside a register: the lane keeps the
+// smaller of itself and its partner iff keepMin (one 64-bit compare).
 __device__ __forceinline__ uint64_t cmpXchg(uint64_t x, int j, bool keepMin, int lane) {
     const uint64_t p = xorLane64(x, j, lane);
-    const uint64_t mn = x < p ? x : p;
-    const uint64_t mx = x < p ? p : x;
-    return keepMin ? mn : mx;
+    return ((x < p) == keepMin) ? x : p;
 }
 
 // Ascending bitonic sort of 64*C keys, element e = c*64 + lane in key[c].
