@@ -116,16 +116,20 @@ def cpu_baseline(base, graph, queries, Ls, k, seconds):
                       f"ParlayANN beam_search"}
 
 
-def homogeneous_batched(b, g, queries, Ls, k, steps, warmup):
-    """The same uniform workload through the batched persistent form (graph-replayed):
-    returns (wall ms per launch, event ms per launch, results)."""
+def homogeneous_batched(b, g, queries, Ls, k, steps, warmup, tune=True):
+    """The same uniform workload through the batched persistent form (graph-replayed), in
+    the shape the autotuner picks for it: returns (wall ms per launch, event ms per launch,
+    results, shape, resident waves, autotune seconds)."""
     from pann import _lib, search
     import ctypes as C
     L = _lib.lib()
     nq = len(queries)
     params = search.make_params(nq, Ls, k)
+    t = time.time()
+    cfg = search.autotune(queries, params, Ls, k, base=b, graph=g)[0] if tune else None
+    tune_s = time.time() - t
     st = search.Stream()
-    batch = search.Batch(nq, Ls, k, base=b, graph=g)
+    batch = search.Batch(nq, Ls, k, base=b, graph=g, config=cfg)
     batch.set_queries(queries, params, st.ptr)
     batch.launch(st.ptr)
     batch.capture(st.ptr)
@@ -155,7 +159,7 @@ def homogeneous_batched(b, g, queries, Ls, k, steps, warmup):
     cfg, waves = batch.config()
     batch.free()
     st.free()
-    return wall, tot / steps, r, cfg, waves
+    return wall, tot / steps, r, cfg, waves, tune_s
 
 
 def mixed_workload(queries, nq_total, seed=4):
@@ -305,6 +309,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the batched-vs-plain workload")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="time the batched form in its default shape instead of the autotuned one")
     ap.add_argument("--mixed-nq", type=int, default=65_536)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-rates", default="200000,1000000,2000000")
@@ -361,7 +367,8 @@ def main():
     plain_wall, plain_ev = time_launches(s, b, q, g, args.steps, args.warmup)
     log(f"[bench] plain Ls={Ls}: {plain_wall:.3f} ms/launch wall, {plain_ev:.3f} ms kernel, "
         f"{args.nq / plain_wall * 1e3:,.0f} QPS")
-    bat_wall, bat_ev, rbat, bcfg, bwaves = homogeneous_batched(b, g, queries, Ls, k, args.steps, args.warmup)
+    bat_wall, bat_ev, rbat, bcfg, bwaves, btune = homogeneous_batched(b, g, queries, Ls, k, args.steps,
+                                                                       args.warmup, not args.no_tune)
     same = bool(np.array_equal(rbat.ids[:, :k], r.ids) and np.array_equal(rbat.comps, r.comps))
     log(f"[bench] batched Ls={Ls}: {bat_wall:.3f} ms/launch wall, {bat_ev:.3f} ms kernel, "
         f"{args.nq / bat_wall * 1e3:,.0f} QPS, identical to plain: {same}")
@@ -424,7 +431,9 @@ def main():
             "batched_identical_to_plain": same,
             "batched_shape": {"waves_per_block": bcfg["waves_per_block"],
                               "lds_filter_bits": bcfg["lds_filter_bits"],
-                              "visited_slots": bcfg["visited_slots"], "resident_waves": bwaves},
+                              "visited_slots": bcfg["visited_slots"], "prefetch": bcfg["prefetch"],
+                              "resident_waves": bwaves, "autotuned": not args.no_tune,
+                              "autotune_s": round(btune, 2)},
             "kernel_ms": round(ev_ms, 4), "comps_per_query": round(float(r.comps.mean()), 1),
             "edges_per_query": round(float(r.edges.mean()), 1),
             "hops_per_query": round(float(r.hops.mean()), 1),

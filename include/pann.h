@@ -102,14 +102,16 @@ int pann_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
 /* Per-section cycle sums of the flat search, as the reference's -DPERFIL build keeps
  * them (cuda/busqueda.cu:15-32): init, visit, filter, distances, sort, union, merge,
  * cut, next, output, then the hop and query counts, then (register-resident form) the
- * candidates below the cutoff summed over hops, the hops with none and the hops with
- * 1..8 of them.  Only a library built with
+ * candidates below the cutoff summed over hops, the hops with none, the hops with 1..8
+ * of them, and the hops whose next node's row had been prefetched.  Only a library
+ * built with
  * `build.py --profile` (libpann_prof.so) collects them; others return PANN_ERR_CAPACITY. */
 int pann_profile_counters(uint64_t* out, int n, int reset);
 
 /* ---- vectors (VectoresGPU) ---- */
 /* subirVectores (cuda/datos_gpu.cu:20-40): host rows of `dim` int8 -> device rows with a
- * 128-byte pitch, zero padded.  dim <= 128. */
+ * 128-byte pitch, zero padded.  dim <= 128.  The object also keeps each row's squared
+ * norm (used by pann_bruteforce_knn); pann_vectors_copy_async refreshes them. */
 int pann_vectors_create(const int8_t* host_rows, uint32_t n, uint32_t dim, pann_vectors** out);
 /* leerVectoresI8bin (cuda/lectores.cpp:70-95) followed by subirVectores. */
 int pann_vectors_load(const char* i8bin_path, pann_vectors** out);
@@ -179,7 +181,8 @@ typedef struct pann_batch_config {
     int32_t waves_per_block; /* 1, 2 or 4 (0: the most that fit 64 KB of LDS) */
     int32_t lds_filter_bits; /* seen filters up to 2^bits slots live in LDS, 10..12 (0: 12) */
     uint32_t visited_slots;  /* LDS visited / is_inw hash slots, power of two >= 64 (0: derived) */
-    uint32_t reserved;
+    uint32_t prefetch;       /* next-row prefetch depth, 0 or 1: while a hop merges, load the
+                                row of the predicted next node (flat graphs, max_Ls <= 64) */
 } pann_batch_config;
 
 /* Allocates device buffers for up to max_queries queries over a flat graph or HNSW. */

@@ -381,8 +381,19 @@ constexpr int kInsertMax = 16;  // up to this many candidates are inserted one b
 __device__ __forceinline__ uint64_t tagKey(int d, uint32_t id, uint32_t tag) {
     return ((uint64_t)(uint32_t)d << 40) | ((uint64_t)id << 8) | tag;
 }
-__device__ __forceinline__ int keyDist(uint64_t k) { return (int)(k >> 40); }
-__device__ __forceinline__ uint32_t keyId(uint64_t k) { return (uint32_t)(k >> 8); }
+__device__ __forceinline__ int keyDist(uint64_t k) { return (int)((uint32_t)(k >> 32) >> 8); }
+__device__ __forceinline__ uint32_t keyId(uint64_t k) {
+    return ((uint32_t)(k >> 32) << 24) | ((uint32_t)k >> 8);
+}
+
+// Row `i` of a table at byte offset i * pitch + extra: a 32-bit offset from a scalar base
+// when the table is below 4 GiB (NARROW), else 64-bit address arithmetic.
+template <bool NARROW, typename T>
+__device__ __forceinline__ const T* rowAt(const void* base, uint32_t i, uint32_t pitch, uint32_t extra) {
+    const char* b = static_cast<const char*>(base);
+    if constexpr (NARROW) return reinterpret_cast<const T*>(b + (i * pitch + extra));
+    else return reinterpret_cast<const T*>(b + ((size_t)i * pitch + extra));
+}
 
 // Value of lane 8 * (lane / 8) + r (ds_swizzle bit mode: and 0x18, or r).
 __device__ __forceinline__ uint32_t groupBroadcast8(uint32_t x, int r) {
@@ -457,6 +468,7 @@ __device__ __forceinline__ UnionMarks unionMarks(uint64_t& x, uint64_t prev, int
     return m;
 }
 
+template <bool NARROW, bool PREFETCH>
 __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, const QueryCfg& c,
                            Prof& P) {
     const int lane = laneId();
@@ -497,13 +509,17 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
     int comps = 1, edges = 0, hops = 0, flags = 0;
     uint32_t distinct = 0;
     const uint32_t G = L.G;
+    uint32_t predId = kEmpty;  // node whose row was prefetched into aNext
+    uint32_t aNext = kEmpty;
     __builtin_amdgcn_wave_barrier();
     P.mark(kProfInit);
     P.count(kProfQueries);
 
     while (u >= 0 && visits < c.limit) {
         const uint32_t uid = keyId(readlane64(F, u));
-        const uint32_t a = (uint32_t)lane < G ? L.adj[(size_t)uid * G + lane] : kEmpty;
+        uint32_t a = aNext;
+        if (!PREFETCH || uid != predId)
+            a = (uint32_t)lane < G ? *rowAt<NARROW, uint32_t>(L.adj, uid, 4 * G, 4 * (uint32_t)lane) : kEmpty;
 
         // visit u: its flag, and its expansion count (64 probe slots read at once)
         if (lane == u) F |= 1u;
@@ -560,8 +576,7 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             v[r] = make_int4(0, 0, 0, 0);
-            if ((gmask >> r) & 1u)
-                v[r] = *reinterpret_cast<const int4*>(L.base + (size_t)ids[r] * kVecBytes + sub * 16);
+            if ((gmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
         }
 
         // expansions of the id so far (decide the visited flags of entering copies)
@@ -590,17 +605,48 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
             const uint32_t lmask = (uint32_t)(late >> (8 * grp)) & 0xFFu;
 #pragma unroll
             for (int r = 0; r < 8; r++)
-                if ((lmask >> r) & 1u)
-                    v[r] = *reinterpret_cast<const int4*>(L.base + (size_t)ids[r] * kVecBytes + sub * 16);
+                if ((lmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
         }
         P.mark(kProfFilter);
 
         int part8[8];
 #pragma unroll
         for (int r = 0; r < 8; r++) part8[r] = slicePartial(v[r], qv);
+        // p.p is recomputed from the row: a stored norm would cost one more random
+        // cache line per candidate
         const int d = qq + transposeReduce8(part8, lane);  // distance of entry `lane`
         const uint64_t C = (keep && d < cutoff) ? tagKey(d, a, 0x80u | cnt) : kNoKey;
         P.mark(kProfDist);
+
+        // Prefetch the row of the likely next node: the better (by distance) of the next
+        // unvisited frontier entry and the closest candidate never expanded.  A wrong guess
+        // costs one row; the next hop then loads its own.
+        if constexpr (PREFETCH) {
+            const uint64_t unv = ballot(lane < fsize && !(F & 1u));
+            int dF = 0x7FFFFFFF;
+            predId = kEmpty;
+            if (unv) {
+                const uint64_t f = readlane64(F, __ffsll((unsigned long long)unv) - 1);
+                dF = keyDist(f);
+                predId = keyId(f);
+            }
+            const bool fresh = C != kNoKey && (C & 0x7Fu) == 0;
+            uint32_t dd = fresh ? (uint32_t)d : 0xFFFFFFFFu;
+            dd = min(dd, dppMov<0xB1>(dd));   // quad
+            dd = min(dd, dppMov<0x4E>(dd));
+            dd = min(dd, dppMov<0x124>(dd));  // row: rotate by 4, then by 8
+            dd = min(dd, dppMov<0x128>(dd));
+            const uint32_t m = min(min((uint32_t)readlane((int)dd, 0), (uint32_t)readlane((int)dd, 16)),
+                                   min((uint32_t)readlane((int)dd, 32), (uint32_t)readlane((int)dd, 48)));
+            if (m < (uint32_t)dF) {
+                const uint64_t w = ballot(fresh && (uint32_t)d == m);
+                predId = (uint32_t)readlane((int)a, __ffsll((unsigned long long)w) - 1);
+            }
+            aNext = kEmpty;
+            if (predId != kEmpty && (uint32_t)lane < G)
+                aNext = *rowAt<NARROW, uint32_t>(L.adj, predId, 4 * G, 4 * (uint32_t)lane);
+        }
+
         const uint64_t candM = ballot(C != kNoKey);
         const int nc = __popcll(candM);
         P.add(kProfCands, (unsigned long long)nc);
@@ -609,8 +655,8 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
 
         bool changed = false;
         if (nc > 0 && nc <= kInsertMax) {
-            // few candidates: insert each at its upper bound (equal keys: frontier copies,
-            // then the candidate copies already inserted), trimming to Ls as we go
+            // insert each at its upper bound (equal keys: frontier copies, then the
+            // candidate copies already inserted), trimming to Ls as we go
             int fs = fsize;
             uint64_t mm = candM;
             while (mm) {
@@ -621,22 +667,22 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
                 const int pos = __popcll(ballot(F <= xmax));
                 if (pos >= Ls) continue;
                 const int lower = __popcll(ballot(F < xmin));
-                const int cnt = (int)(x & 0x7Fu);
-                uint32_t vis;
-                if (pos == lower) {
-                    // no copy of the key yet, so no earlier candidate copy either (it
-                    // would have been inserted, or dropped against a frontier copy)
-                    vis = cnt > 0 ? 1u : 0u;
-                } else {
-                    // rep counts the earlier candidate copies, inserted or dropped
-                    const int cF = __popcll(ballot(F >= xmin && F <= xmax && !(F & 0x80u)));
-                    const int rep = __popcll(ballot(C == x) & ((1ull << j) - 1));
-                    if (rep < cF) continue;
-                    vis = rep < cnt ? 1u : 0u;
+                // cF frontier copies of the key; rep earlier candidate copies (inserted or
+                // dropped).  Without any copy yet there is no earlier candidate copy either:
+                // it would have been inserted, or dropped against a frontier copy.
+                int cF = 0, rep = 0;
+                if (pos != lower) {
+                    cF = __popcll(ballot(F >= xmin && F <= xmax && !(F & 0x80u)));
+                    rep = __popcll(ballot(C == x) & ((1ull << j) - 1));
                 }
-                const uint64_t xn = xmin | 0x80u | vis;
-                const uint64_t pv = prevLane64(F, kNoKey, lane);
-                F = lane < pos ? F : (lane == pos ? xn : pv);
+                if (rep < cF) continue;
+                const uint32_t vis = rep < (int)(x & 0x7Fu) ? 1u : 0u;  // rep < expansions
+                // shift the tail up one lane, then place the key (lane 0 never takes the
+                // shifted value: pos >= 0)
+                const uint64_t pv = ((uint64_t)dppMov<0x138>((uint32_t)(F >> 32)) << 32) |
+                                    dppMov<0x138>((uint32_t)F);
+                if (lane > pos) F = pv;
+                if (lane == pos) F = xmin | 0x80u | vis;
                 fs = min(Ls, fs + 1);
                 changed = true;
             }
@@ -696,6 +742,9 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
 
         const uint64_t nv = ballot(lane < fsize && !(F & 1u));
         u = nv ? __ffsll((unsigned long long)nv) - 1 : -1;
+#ifdef PANN_PROFILE
+        if (u >= 0 && keyId(readlane64(F, u)) == predId) P.count(kProfPredFC);
+#endif
         P.mark(kProfNext);
     }
 
@@ -722,10 +771,12 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
     P.mark(kProfOutput);
 }
 
-// FAST: every query of the launch takes searchFast (Ls <= 64, rows <= 64, layout
-// flatFastLdsLayout); otherwise queries with Ls <= 64 on rows of at most 64 entries take
-// it when the launch holds a filter of at least kFastFilterBits in LDS.
-template <int MAXCH, bool FAST>
+// FAST (1: 64-bit addressing, 2: tables below 4 GiB, 3 / 4: the same with next-row
+// prefetch): every query of the launch takes searchFast (Ls <= 64, rows <= 64, layout
+// flatFastLdsLayout).  FAST 0: queries with
+// Ls <= 64 on rows of at most 64 entries take it when the launch holds a filter of at
+// least kFastFilterBits in LDS; the others take searchOne.
+template <int MAXCH, int FAST>
 __global__ __launch_bounds__(256) void beamFlatKernel(FlatLaunch L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = laneId();
@@ -756,18 +807,18 @@ __global__ __launch_bounds__(256) void beamFlatKernel(FlatLaunch L) {
         }
         makeUniform(c);
         const int fb = flatFilterBits(c.Ls);
-        if constexpr (FAST) {
+        if constexpr (FAST != 0) {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride) {
                 if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
             } else {
-                searchFast(L, lds, q, c, P);
+                searchFast<FAST == 2 || FAST == 4, (FAST >= 3)>(L, lds, q, c, P);
             }
         } else {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride ||
                 (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
                 if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
             } else if (MAXCH == 1 && c.Ls <= kFastLsMax && L.fbitsLds >= kFastFilterBits) {
-                searchFast(L, lds, q, c, P);
+                searchFast<false, false>(L, lds, q, c, P);
             } else if (fb <= L.fbitsLds) {
                 searchOne<MAXCH, false>(L, lds, q, c, nullptr, P);
             } else {
@@ -807,30 +858,41 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
     const dim3 grid(blocks), block(64 * wavesPerBlock);
     if (L.fast) {
         if (maxch != 1 || L.LsMax > kFastLsMax) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((beamFlatKernel<1, true>), grid, block, lds, stream, L);
+        switch (L.fast) {
+            case 1: hipLaunchKernelGGL((beamFlatKernel<1, 1>), grid, block, lds, stream, L); break;
+            case 2: hipLaunchKernelGGL((beamFlatKernel<1, 2>), grid, block, lds, stream, L); break;
+            case 3: hipLaunchKernelGGL((beamFlatKernel<1, 3>), grid, block, lds, stream, L); break;
+            case 4: hipLaunchKernelGGL((beamFlatKernel<1, 4>), grid, block, lds, stream, L); break;
+            default: return hipErrorInvalidValue;
+        }
         return hipGetLastError();
     }
     switch (maxch) {
-        case 1: hipLaunchKernelGGL((beamFlatKernel<1, false>), grid, block, lds, stream, L); break;
-        case 2: hipLaunchKernelGGL((beamFlatKernel<2, false>), grid, block, lds, stream, L); break;
-        case 4: hipLaunchKernelGGL((beamFlatKernel<4, false>), grid, block, lds, stream, L); break;
+        case 1: hipLaunchKernelGGL((beamFlatKernel<1, 0>), grid, block, lds, stream, L); break;
+        case 2: hipLaunchKernelGGL((beamFlatKernel<2, 0>), grid, block, lds, stream, L); break;
+        case 4: hipLaunchKernelGGL((beamFlatKernel<4, 0>), grid, block, lds, stream, L); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t beamFlatOccupancy(int maxch, bool fast, int wavesPerBlock, size_t ldsBytes,
+hipError_t beamFlatOccupancy(int maxch, int fast, int wavesPerBlock, size_t ldsBytes,
                              int* blocksPerCU) {
     const int threads = 64 * wavesPerBlock;
     if (fast) {
         if (maxch != 1) return hipErrorInvalidValue;
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, true>,
-                                                            threads, ldsBytes);
+        switch (fast) {
+            case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, 1>, threads, ldsBytes);
+            case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, 2>, threads, ldsBytes);
+            case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, 3>, threads, ldsBytes);
+            case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, 4>, threads, ldsBytes);
+            default: return hipErrorInvalidValue;
+        }
     }
     switch (maxch) {
-        case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, false>, threads, ldsBytes);
-        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<2, false>, threads, ldsBytes);
-        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<4, false>, threads, ldsBytes);
+        case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, 0>, threads, ldsBytes);
+        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<2, 0>, threads, ldsBytes);
+        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<4, 0>, threads, ldsBytes);
         default: return hipErrorInvalidValue;
     }
 }

@@ -15,6 +15,7 @@ enum ProfSection {
     kProfCands,    // candidates below the cutoff, summed over hops
     kProfNoCands,  // hops with no candidate below the cutoff
     kProfSmall,    // hops with 1..8 candidates below the cutoff
+    kProfPredFC,   // hops whose next node was the prefetched one
     kProfSections
 };
 #ifdef PANN_PROFILE

@@ -199,12 +199,18 @@ int pann_vectors_create(const int8_t* host_rows, uint32_t n, uint32_t dim, pann_
     const size_t bytes = std::max<size_t>(kVecBytes, (size_t)n * kVecBytes);
     hipError_t e = hipMalloc(&v->dev, bytes);
     if (e == hipSuccess) e = hipMemset(v->dev, 0, bytes);
+    if (e == hipSuccess) e = hipMalloc(&v->norms, 4 * std::max<size_t>(1, n));
+    if (e == hipSuccess) e = hipMemset(v->norms, 0, 4 * std::max<size_t>(1, n));
     if (e != hipSuccess) {
-        hipFree(v->dev);
-        delete v;
+        pann_vectors_free(v);
         return setError(PANN_ERR_HIP, "allocating %zu bytes of vectors: %s", bytes, hipGetErrorString(e));
     }
     int r = uploadRows(v->dev, host_rows, n, dim, nullptr, false);
+    if (r == PANN_OK) {
+        e = launchRowNorms(v->dev, n, v->norms, nullptr);
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess) r = setError(PANN_ERR_HIP, "row norms: %s", hipGetErrorString(e));
+    }
     if (r != PANN_OK) {
         pann_vectors_free(v);
         return r;
@@ -224,7 +230,10 @@ int pann_vectors_load(const char* path, pann_vectors** out) {
 int pann_vectors_copy_async(pann_vectors* v, const int8_t* host_rows, uint32_t n, void* stream) {
     if (!v || (!host_rows && n)) return setError(PANN_ERR_ARG, "NULL argument");
     if (n > v->n) return setError(PANN_ERR_ARG, "%u rows do not fit in %u", n, v->n);
-    return uploadRows(v->dev, host_rows, n, v->dim, (hipStream_t)stream, true);
+    const int r = uploadRows(v->dev, host_rows, n, v->dim, (hipStream_t)stream, true);
+    if (r != PANN_OK) return r;
+    HIP_TRY(launchRowNorms(v->dev, n, v->norms, (hipStream_t)stream));
+    return PANN_OK;
 }
 
 int pann_vectors_info(const pann_vectors* v, uint32_t* n, uint32_t* dim, const int8_t** rows,
@@ -240,6 +249,7 @@ int pann_vectors_info(const pann_vectors* v, uint32_t* n, uint32_t* dim, const i
 void pann_vectors_free(pann_vectors* v) {
     if (!v) return;
     hipFree(v->dev);
+    hipFree(v->norms);
     delete v;
 }
 
@@ -454,7 +464,7 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
     s->maxch = maxch;
     s->wavesPerBlock = (int)std::min<size_t>(4, std::max<size_t>(1, kMaxLdsPerBlock / lay.total));
     int bpc = 0, cus = 0;
-    hipError_t he = beamFlatOccupancy(maxch, fast, s->wavesPerBlock, (size_t)lay.total * s->wavesPerBlock, &bpc);
+    hipError_t he = beamFlatOccupancy(maxch, fast ? fastMode(g->n, g->G) : 0, s->wavesPerBlock, (size_t)lay.total * s->wavesPerBlock, &bpc);
     r = deviceCUs(&cus);
     if (he != hipSuccess || r != PANN_OK) {
         delete s;
@@ -478,7 +488,7 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
     L.capV = capV;
     L.ldsPerWave = lay.total;
     L.lds = lay;
-    L.fast = fast ? 1 : 0;
+    L.fast = fast ? fastMode(g->n, g->G) : 0;
     L.totalWaves = totalWaves;
     L.queue = nullptr;
     if (!ldsFilter) {
@@ -660,16 +670,9 @@ int pann_bruteforce_knn(const pann_vectors* base, const pann_vectors* queries, u
     if (k == 0 || k > 128) return setError(PANN_ERR_CAPACITY, "k=%u outside 1..128", k);
     if (nq == 0) return PANN_OK;
     const hipStream_t st = (hipStream_t)stream;
-    int32_t *bn = nullptr, *qn = nullptr;
-    hipError_t e = hipMalloc(&bn, 4ull * std::max<uint32_t>(1, base->n));
-    if (e == hipSuccess) e = hipMalloc(&qn, 4ull * nq);
-    if (e == hipSuccess) e = launchRowNorms(base->dev, base->n, bn, st);
-    if (e == hipSuccess) e = launchRowNorms(queries->dev, nq, qn, st);
-    if (e == hipSuccess)
-        e = launchBruteForceKnn(base->dev, bn, base->n, queries->dev, qn, nq, k, ids_dev, dists_dev, st);
+    hipError_t e = launchBruteForceKnn(base->dev, base->norms, base->n, queries->dev, queries->norms,
+                                       nq, k, ids_dev, dists_dev, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(bn);
-    hipFree(qn);
     if (e != hipSuccess) return setError(PANN_ERR_HIP, "brute-force k-NN: %s", hipGetErrorString(e));
     return PANN_OK;
 }

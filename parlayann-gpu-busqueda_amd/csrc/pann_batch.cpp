@@ -56,6 +56,7 @@ int checkBatchConfig(const pann_batch_config* cfg) {
     if (cfg->visited_slots &&
         ((cfg->visited_slots & (cfg->visited_slots - 1)) || cfg->visited_slots < 64))
         return setError(PANN_ERR_ARG, "visited_slots must be 0 or a power of two >= 64");
+    if (cfg->prefetch > 1) return setError(PANN_ERR_ARG, "prefetch must be 0 or 1");
     return PANN_OK;
 }
 
@@ -85,7 +86,9 @@ int finishSizing(pann_batch* b, uint32_t ldsPerWave, int gbits) {
     b->cfg.waves_per_block = wpb;
     int bpc = 0;
     hipError_t e = b->kind == 0
-                       ? beamFlatOccupancy(b->maxch, b->fast, wpb, (size_t)ldsPerWave * wpb, &bpc)
+                       ? beamFlatOccupancy(b->maxch,
+                                           b->fast ? fastMode(b->graph->n, b->graph->G) + (b->cfg.prefetch ? 2 : 0) : 0,
+                                           wpb, (size_t)ldsPerWave * wpb, &bpc)
                        : beamHnswOccupancy(b->wr, b->maxch, wpb, (size_t)ldsPerWave * wpb, &bpc);
     if (e != hipSuccess) return setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
     b->blocks = std::max(1, bpc) * cus;
@@ -122,6 +125,7 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
     b->cfg.visited_slots = capV;
     const bool fast = b->maxch == 1 && LsMax <= kFastLsMax && fbLds >= kFastFilterBits;
     b->fast = fast;
+    if (!fast) b->cfg.prefetch = 0;  // only the register-resident form prefetches
     const FlatLds lay = fast ? flatFastLdsLayout(capV) : flatLdsLayout(LsMax, b->maxch, capV, fbLds);
     if (lay.total > kMaxLdsPerBlock) {
         delete b;
@@ -142,7 +146,7 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
     L.capV = capV;
     L.ldsPerWave = lay.total;
     L.lds = lay;
-    L.fast = fast ? 1 : 0;
+    L.fast = fast ? fastMode(g->n, g->G) + (cfg->prefetch ? 2 : 0) : 0;
     L.gfilt = b->gfilt;
     L.queue = b->queue;
     L.order = b->order;
@@ -178,6 +182,7 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
     b->kind = 1;
     b->hnsw = h;
     b->cfg = *cfg;
+    b->cfg.prefetch = 0;  // only the flat register-resident form prefetches
     b->maxQueries = max_queries;
     b->dim = h->dim;
     b->wr = wr;
@@ -371,7 +376,10 @@ int pann_batch_autotune(const pann_vectors* base, const pann_graph* g, const pan
             for (uint32_t vs : {derived, derived / 2, derived * 2}) {
                 if (vs < (h ? 512u : 256u) || vs > 16384) continue;
                 if (!h && vs < 2u * (uint32_t)max_Ls + 64) continue;
-                cands.push_back({max_Ls, max_k, wpb, fb, vs, 0});
+                for (uint32_t pf : {0u, 1u}) {
+                    if (pf && (h || max_Ls > kFastLsMax)) continue;
+                    cands.push_back({max_Ls, max_k, wpb, fb, vs, pf});
+                }
             }
         }
     hipStream_t st = nullptr;

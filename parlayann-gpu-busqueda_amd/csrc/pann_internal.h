@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 namespace pann {
 
@@ -70,6 +71,16 @@ __host__ __device__ inline FlatLds flatLdsLayout(int LsMax, int maxch, uint32_t 
 constexpr int kFastLsMax = 64;
 constexpr int kFastFilterBits = 10;  // flatFilterBits(Ls) for every Ls <= 64
 
+// Value of FlatLaunch::fast for a register-resident launch over n rows of G entries.
+// PANN_WIDE_ADDRESSING=1 in the environment selects 64-bit addressing for any size (so
+// that form can be tested on small indexes).
+inline int fastMode(uint32_t n, uint32_t G) {
+    const char* w = getenv("PANN_WIDE_ADDRESSING");
+    if (w && w[0] == '1') return 1;
+    const uint64_t vecBytes = (uint64_t)n * kVecBytes, adjBytes = (uint64_t)n * G * 4;
+    return (vecBytes <= 0xFFFFFFFFull && adjBytes <= 0xFFFFFFFFull) ? 2 : 1;
+}
+
 __host__ __device__ inline FlatLds flatFastLdsLayout(uint32_t capV) {
     FlatLds L = {};
     uint32_t o = 0;
@@ -114,7 +125,9 @@ struct FlatLaunch {
     uint32_t capV;             // visited hash capacity in LDS (power of two)
     uint32_t ldsPerWave;
     FlatLds lds;
-    int32_t fast;              // 1: register-resident kernel, layout flatFastLdsLayout
+    int32_t fast;              // register-resident kernel (layout flatFastLdsLayout): 1, or
+                               // 2 when base and adjacency are below 4 GiB; +2 with next-row
+                               // prefetch; 0: general
     uint32_t* gfilt;           // per-wave global filter regions
     uint32_t* queue;           // persistent mode: {next ticket, finished waves}; null: static
     const uint32_t* order;     // persistent mode: ticket -> query (null: identity)
@@ -191,7 +204,7 @@ hipError_t beamHnswOccupancy(int wr, int maxch, int wavesPerBlock, size_t ldsByt
 // Launchers implemented in the .hip translation units.
 hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int blocks,
                           hipStream_t stream);
-hipError_t beamFlatOccupancy(int maxch, bool fast, int wavesPerBlock, size_t ldsBytes,
+hipError_t beamFlatOccupancy(int maxch, int fast, int wavesPerBlock, size_t ldsBytes,
                              int* blocksPerCU);
 hipError_t launchRowNorms(const int8_t* rows, uint32_t n, int32_t* out, hipStream_t s);
 // Per-section cycle counters of the flat search (profiling builds only).

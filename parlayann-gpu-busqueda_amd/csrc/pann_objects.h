@@ -12,7 +12,8 @@ using pann::HnswLaunch;
 
 struct pann_vectors {
     uint32_t n = 0, dim = 0;
-    int8_t* dev = nullptr;  // n x kVecBytes
+    int8_t* dev = nullptr;     // n x kVecBytes
+    int32_t* norms = nullptr;  // n squared norms (p . p), kept in step with the rows
 };
 
 struct pann_graph {

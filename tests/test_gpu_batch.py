@@ -146,3 +146,32 @@ def test_autotune_returns_clean_config(gpu, small):
     search.synchronize()
     assert (batch.download().flags & 5 == 0).all()
     batch.free(), g.free(), b.free()
+
+
+@pytest.mark.parametrize("prefetch,wide", [(0, False), (1, False), (1, True)])
+def test_register_resident_batch_forms(gpu, w1, monkeypatch, prefetch, wide):
+    """Batches with max_Ls <= 64 run the register-resident kernel: with and without the
+    next-row prefetch, with 32-bit and 64-bit addressing, on uneven Ls / k / visit limits
+    and on a graph with duplicate edges."""
+    from oracle import oracle as O
+    from pann import datagen, search
+    from conftest import random_graph
+    if wide:
+        monkeypatch.setenv("PANN_WIDE_ADDRESSING", "1")
+    rng = np.random.default_rng(11 + prefetch)
+    base, queries, graph = w1
+    cases = [(base, queries, graph)]
+    b2, q2 = datagen.gmm_dataset(3000, 150, dim=100, ncomp=6, seeds=(21, 22, 23))
+    cases.append((b2, q2, random_graph(3000, 64, np.random.default_rng(64), dup_frac=0.2, min_degree=21)))
+    for base, queries, graph in cases:
+        params = _mixed_params(len(queries), len(base), rng, ls_lo=1, ls_hi=64, ks=(1, 10, 50))
+        b, g = search.Vectors(base), search.FlatGraph(graph)
+        batch = search.Batch(len(queries), 64, 50, base=b, graph=g, prefetch=prefetch)
+        assert batch.config()[0]["prefetch"] == prefetch
+        batch.set_queries(queries, params)
+        batch.launch()
+        search.synchronize()
+        r = batch.download()
+        o = O.beam_flat(base, graph, queries, _oracle_params(params))
+        _check_against_oracle(r, o, params, f"prefetch={prefetch} wide={wide}")
+        batch.free(), g.free(), b.free()

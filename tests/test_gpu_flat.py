@@ -123,3 +123,21 @@ def test_determinism_repeat_launch(gpu, small):
     assert (first.ids == last.ids).all() and (first.dists == last.dists).all()
     assert (first.comps == last.comps).all()
     s.free(), g.free(), q.free(), b.free()
+
+
+def test_wide_addressing_form(gpu, w1, monkeypatch):
+    """The register-resident search with 64-bit address arithmetic (the form used when the
+    vectors or the adjacency exceed 4 GiB), forced on small indexes."""
+    monkeypatch.setenv("PANN_WIDE_ADDRESSING", "1")
+    base, queries, graph = w1
+    for Ls in (10, 40):
+        r = _run(base, graph, queries, Ls, 10)
+        o = _oracle(base, graph, queries, Ls, 10)
+        _assert_same(r, o, 10, f"wide W1 Ls={Ls}")
+    from pann import datagen
+    rng = np.random.default_rng(64)
+    base, queries = datagen.gmm_dataset(3000, 150, dim=100, ncomp=6, seeds=(21, 22, 23))
+    graph = random_graph(3000, 64, rng, dup_frac=0.2, min_degree=21)
+    r = _run(base, graph, queries, 33, 10)
+    o = _oracle(base, graph, queries, 33, 10)
+    _assert_same(r, o, 10, "wide dup G=64 Ls=33")
