@@ -773,11 +773,11 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
 
 // FAST (1: 64-bit addressing, 2: tables below 4 GiB, 3 / 4: the same with next-row
 // prefetch): every query of the launch takes searchFast (Ls <= 64, rows <= 64, layout
-// flatFastLdsLayout).  FAST 0: queries with
-// Ls <= 64 on rows of at most 64 entries take it when the launch holds a filter of at
-// least kFastFilterBits in LDS; the others take searchOne.
+// flatFastLdsLayout).  FAST 0: queries with Ls <= 64 on rows of at most 64 entries take
+// it when the launch holds a filter of at least kFastFilterBits in LDS; the others take
+// searchOne.
 template <int MAXCH, int FAST>
-__global__ __launch_bounds__(256) void beamFlatKernel(FlatLaunch L) {
+__device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = laneId();
     const int waveInBlock = threadIdx.x >> 6;
@@ -831,6 +831,19 @@ __global__ __launch_bounds__(256) void beamFlatKernel(FlatLaunch L) {
     if (L.queue) finishQueue(L.queue, L.totalWaves, lane);
 }
 
+template <int MAXCH>
+__global__ __launch_bounds__(256) void beamFlatKernel(FlatLaunch L) {
+    drainQueries<MAXCH, 0>(L);
+}
+
+// The register-resident kernel is held to 96 VGPRs so five wavefronts share a SIMD (its
+// LDS, 7.7 KB per wavefront at Ls <= 25, allows as many): the hop is a chain of
+// dependent gathers and LDS round trips, and the extra wavefront hides them.
+template <int FAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void beamFlatFastKernel(FlatLaunch L) {
+    drainQueries<1, FAST>(L);
+}
+
 }  // namespace
 
 #ifdef PANN_PROFILE
@@ -859,18 +872,18 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
     if (L.fast) {
         if (maxch != 1 || L.LsMax > kFastLsMax) return hipErrorInvalidValue;
         switch (L.fast) {
-            case 1: hipLaunchKernelGGL((beamFlatKernel<1, 1>), grid, block, lds, stream, L); break;
-            case 2: hipLaunchKernelGGL((beamFlatKernel<1, 2>), grid, block, lds, stream, L); break;
-            case 3: hipLaunchKernelGGL((beamFlatKernel<1, 3>), grid, block, lds, stream, L); break;
-            case 4: hipLaunchKernelGGL((beamFlatKernel<1, 4>), grid, block, lds, stream, L); break;
+            case 1: hipLaunchKernelGGL((beamFlatFastKernel<1>), grid, block, lds, stream, L); break;
+            case 2: hipLaunchKernelGGL((beamFlatFastKernel<2>), grid, block, lds, stream, L); break;
+            case 3: hipLaunchKernelGGL((beamFlatFastKernel<3>), grid, block, lds, stream, L); break;
+            case 4: hipLaunchKernelGGL((beamFlatFastKernel<4>), grid, block, lds, stream, L); break;
             default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
     }
     switch (maxch) {
-        case 1: hipLaunchKernelGGL((beamFlatKernel<1, 0>), grid, block, lds, stream, L); break;
-        case 2: hipLaunchKernelGGL((beamFlatKernel<2, 0>), grid, block, lds, stream, L); break;
-        case 4: hipLaunchKernelGGL((beamFlatKernel<4, 0>), grid, block, lds, stream, L); break;
+        case 1: hipLaunchKernelGGL((beamFlatKernel<1>), grid, block, lds, stream, L); break;
+        case 2: hipLaunchKernelGGL((beamFlatKernel<2>), grid, block, lds, stream, L); break;
+        case 4: hipLaunchKernelGGL((beamFlatKernel<4>), grid, block, lds, stream, L); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -882,17 +895,17 @@ hipError_t beamFlatOccupancy(int maxch, int fast, int wavesPerBlock, size_t ldsB
     if (fast) {
         if (maxch != 1) return hipErrorInvalidValue;
         switch (fast) {
-            case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, 1>, threads, ldsBytes);
-            case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, 2>, threads, ldsBytes);
-            case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, 3>, threads, ldsBytes);
-            case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, 4>, threads, ldsBytes);
+            case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFastKernel<1>, threads, ldsBytes);
+            case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFastKernel<2>, threads, ldsBytes);
+            case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFastKernel<3>, threads, ldsBytes);
+            case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFastKernel<4>, threads, ldsBytes);
             default: return hipErrorInvalidValue;
         }
     }
     switch (maxch) {
-        case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1, 0>, threads, ldsBytes);
-        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<2, 0>, threads, ldsBytes);
-        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<4, 0>, threads, ldsBytes);
+        case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<1>, threads, ldsBytes);
+        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<2>, threads, ldsBytes);
+        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatKernel<4>, threads, ldsBytes);
         default: return hipErrorInvalidValue;
     }
 }

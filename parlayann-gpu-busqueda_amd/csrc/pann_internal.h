@@ -67,7 +67,8 @@ __host__ __device__ inline FlatLds flatLdsLayout(int LsMax, int maxch, uint32_t 
 
 // The register-resident form of the flat search (frontier of at most 64 entries, one
 // register per lane; adjacency rows of at most 64 entries) needs only the seen filter,
-// its byte tags, the expansion counts (visited hash) and a 64-entry compaction buffer.
+// its byte tags, the expansion counts (visited hash) and a 64-entry compaction buffer,
+// which shares the tag bytes (tags live only inside the filter step).
 constexpr int kFastLsMax = 64;
 constexpr int kFastFilterBits = 10;  // flatFilterBits(Ls) for every Ls <= 64
 
@@ -82,12 +83,13 @@ inline int fastMode(uint32_t n, uint32_t G) {
 }
 
 __host__ __device__ inline FlatLds flatFastLdsLayout(uint32_t capV) {
+    static_assert(8 * kFastLsMax <= kTagSlots, "compaction buffer must fit in the tag bytes");
     FlatLds L = {};
     uint32_t o = 0;
-    L.kkey = o; o += 8u * kFastLsMax;
     L.vkey = o; o += 4u * capV;
     L.filt = o; o += 4u << kFastFilterBits;
     L.tag = o;  o += kTagSlots;
+    L.kkey = L.tag;
     L.vcnt = o; o += alignUp(capV, 8);
     L.total = alignUp(o, 16);
     return L;
