@@ -111,7 +111,9 @@ int pann_profile_counters(uint64_t* out, int n, int reset);
 /* ---- vectors (VectoresGPU) ---- */
 /* subirVectores (cuda/datos_gpu.cu:20-40): host rows of `dim` int8 -> device rows with a
  * 128-byte pitch, zero padded.  dim <= 128.  The object also keeps each row's squared
- * norm (used by pann_bruteforce_knn); pann_vectors_copy_async refreshes them. */
+ * norm; when dim <= 124 the norm (int32) is also stored in bytes 124..127 of the device
+ * row, where the search kernels read it with the row.  pann_vectors_copy_async
+ * refreshes the norms of the rows it overwrites. */
 int pann_vectors_create(const int8_t* host_rows, uint32_t n, uint32_t dim, pann_vectors** out);
 /* leerVectoresI8bin (cuda/lectores.cpp:70-95) followed by subirVectores. */
 int pann_vectors_load(const char* i8bin_path, pann_vectors** out);

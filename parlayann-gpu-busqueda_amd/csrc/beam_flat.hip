@@ -75,7 +75,10 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
     const uint32_t vmask = capV - 1;
 
     // ---- query slice: lane holds bytes [16*sub, 16*sub+16) ----
-    const int4 qv = *reinterpret_cast<const int4*>(L.queries + (size_t)q * kVecBytes + sub * 16);
+    // rows may hold |p|^2 in bytes 124..127 (lane 7's .w): masked out of the products
+    const int normMask = (L.normsInRow && sub == 7) ? -1 : 0;
+    int4 qv = *reinterpret_cast<const int4*>(L.queries + (size_t)q * kVecBytes + sub * 16);
+    qv.w &= ~normMask;
     const int qq = groupSum8(sliceNorm(qv));
 
     // ---- fresh tables for this query (the CPU creates them per query) ----
@@ -88,7 +91,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
     }
 
     // ---- start point: node 0 (cuda/busqueda.cu:166-173) ----
-    const int d0 = groupDistance(L.base, 0u, sub, qv, qq);
+    const int d0 = groupDistance(L.base, 0u, sub, qv, qq, normMask);
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) {
         fkey[0][0] = (uint64_t)(uint32_t)d0 << 32;
@@ -185,7 +188,10 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
             }
             int part8[8];
 #pragma unroll
-            for (int r = 0; r < 8; r++) part8[r] = slicePartial(v[r], qv);
+            for (int r = 0; r < 8; r++) {
+                v[r].w &= ~normMask;
+                part8[r] = slicePartial(v[r], qv);
+            }
             const int tot = transposeReduce8(part8, lane);
             const uint32_t il = (uint32_t)sub * 8 + grp;  // candidate held by this lane
             if (il < nb) {
@@ -468,7 +474,7 @@ __device__ __forceinline__ UnionMarks unionMarks(uint64_t& x, uint64_t prev, int
     return m;
 }
 
-template <bool NARROW, bool PREFETCH>
+template <bool NARROW, bool PREFETCH, bool NORMS>
 __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, const QueryCfg& c,
                            Prof& P) {
     const int lane = laneId();
@@ -490,7 +496,10 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
     const uint32_t vmask = capV - 1;
     const uint32_t win = capV < 64 ? capV : 64;
 
-    const int4 qv = *reinterpret_cast<const int4*>(L.queries + (size_t)q * kVecBytes + sub * 16);
+    // NORMS: rows hold |p|^2 in bytes 124..127, lane 7's .w of each 8-lane group
+    const int normMask = (NORMS && sub == 7) ? -1 : 0;
+    int4 qv = *reinterpret_cast<const int4*>(L.queries + (size_t)q * kVecBytes + sub * 16);
+    qv.w &= ~normMask;
     const int qq = groupSum8(sliceNorm(qv));
 
     {
@@ -501,7 +510,7 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
     }
 
     // start point: node 0 (cuda/busqueda.cu:166-173)
-    const int d0 = groupDistance(L.base, 0u, sub, qv, qq);
+    const int d0 = groupDistance(L.base, 0u, sub, qv, qq, normMask);
     uint64_t F = lane == 0 ? tagKey(d0, 0u, 0u) : kNoKey;
     int fsize = 1;
     int u = 0;
@@ -572,10 +581,13 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
         uint32_t ids[8];
 #pragma unroll
         for (int r = 0; r < 8; r++) ids[r] = groupBroadcast8(a, r);
+        // Slices of entries that are not loaded keep whatever the registers hold: their
+        // distances are computed and thrown away.
         int4 v[8];
 #pragma unroll
         for (int r = 0; r < 8; r++) {
-            v[r] = make_int4(0, 0, 0, 0);
+            // an empty asm hands the compiler registers it need not initialise
+            asm("" : "=v"(v[r].x), "=v"(v[r].y), "=v"(v[r].z), "=v"(v[r].w));
             if ((gmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
         }
 
@@ -611,7 +623,8 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
 
         int part8[8];
 #pragma unroll
-        for (int r = 0; r < 8; r++) part8[r] = slicePartial(v[r], qv);
+        for (int r = 0; r < 8; r++)
+            part8[r] = NORMS ? slicePartialN(v[r], qv, normMask) : slicePartial(v[r], qv);
         // p.p is recomputed from the row: a stored norm would cost one more random
         // cache line per candidate
         const int d = qq + transposeReduce8(part8, lane);  // distance of entry `lane`
@@ -669,25 +682,27 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
                 const int lower = __popcll(ballot(F < xmin));
                 // cF frontier copies of the key; rep earlier candidate copies (inserted or
                 // dropped).  Without any copy yet there is no earlier candidate copy either:
-                // it would have been inserted, or dropped against a frontier copy.
-                int cF = 0, rep = 0;
+                // it would have been inserted, or dropped against a frontier copy.  The
+                // entering copy keeps "expansions - rep" in its tag: visited iff positive.
+                uint64_t xe = x;
                 if (pos != lower) {
-                    cF = __popcll(ballot(F >= xmin && F <= xmax && !(F & 0x80u)));
-                    rep = __popcll(ballot(C == x) & ((1ull << j) - 1));
+                    const int cF = __popcll(ballot(F >= xmin && F <= xmax && !(F & 0x80u)));
+                    const int rep = __popcll(ballot(C == x) & ((1ull << j) - 1));
+                    if (rep < cF) continue;
+                    xe = xmin | 0x80u | (uint32_t)max(0, (int)(x & 0x7Fu) - rep);
                 }
-                if (rep < cF) continue;
-                const uint32_t vis = rep < (int)(x & 0x7Fu) ? 1u : 0u;  // rep < expansions
                 // shift the tail up one lane, then place the key (lane 0 never takes the
                 // shifted value: pos >= 0)
                 const uint64_t pv = ((uint64_t)dppMov<0x138>((uint32_t)(F >> 32)) << 32) |
                                     dppMov<0x138>((uint32_t)F);
                 if (lane > pos) F = pv;
-                if (lane == pos) F = xmin | 0x80u | vis;
+                if (lane == pos) F = xe;
                 fs = min(Ls, fs + 1);
                 changed = true;
             }
             if (changed) {
-                F = lane < fs ? ((F & ~0xFFull) | ((uint64_t)lane << 1) | (F & 1u)) : kNoKey;
+                const uint32_t vis = (F & 0x80u) ? ((F & 0x7Fu) ? 1u : 0u) : (uint32_t)(F & 1u);
+                F = lane < fs ? ((F & ~0xFFull) | ((uint64_t)lane << 1) | vis) : kNoKey;
                 fsize = fs;
             }
             P.mark(kProfMerge);
@@ -771,9 +786,9 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
     P.mark(kProfOutput);
 }
 
-// FAST (1: 64-bit addressing, 2: tables below 4 GiB, 3 / 4: the same with next-row
-// prefetch): every query of the launch takes searchFast (Ls <= 64, rows <= 64, layout
-// flatFastLdsLayout).  FAST 0: queries with Ls <= 64 on rows of at most 64 entries take
+// FAST - 1, bit 0: tables below 4 GiB (32-bit offsets), bit 1: next-row prefetch, bit 2:
+// rows carry their squared norm: every query of the launch takes searchFast (Ls <= 64,
+// rows <= 64, layout flatFastLdsLayout).  FAST 0: queries with Ls <= 64 on rows of at most 64 entries take
 // it when the launch holds a filter of at least kFastFilterBits in LDS; the others take
 // searchOne.
 template <int MAXCH, int FAST>
@@ -811,14 +826,15 @@ __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride) {
                 if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
             } else {
-                searchFast<FAST == 2 || FAST == 4, (FAST >= 3)>(L, lds, q, c, P);
+                searchFast<((FAST - 1) & 1) != 0, ((FAST - 1) & 2) != 0, ((FAST - 1) & 4) != 0>(L, lds, q, c, P);
             }
         } else {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride ||
                 (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
                 if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
             } else if (MAXCH == 1 && c.Ls <= kFastLsMax && L.fbitsLds >= kFastFilterBits) {
-                searchFast<false, false>(L, lds, q, c, P);
+                if (L.normsInRow) searchFast<false, false, true>(L, lds, q, c, P);
+                else searchFast<false, false, false>(L, lds, q, c, P);
             } else if (fb <= L.fbitsLds) {
                 searchOne<MAXCH, false>(L, lds, q, c, nullptr, P);
             } else {
@@ -871,11 +887,16 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
     const dim3 grid(blocks), block(64 * wavesPerBlock);
     if (L.fast) {
         if (maxch != 1 || L.LsMax > kFastLsMax) return hipErrorInvalidValue;
-        switch (L.fast) {
+        const int mode = L.fast + (L.normsInRow ? 4 : 0);
+        switch (mode) {
             case 1: hipLaunchKernelGGL((beamFlatFastKernel<1>), grid, block, lds, stream, L); break;
             case 2: hipLaunchKernelGGL((beamFlatFastKernel<2>), grid, block, lds, stream, L); break;
             case 3: hipLaunchKernelGGL((beamFlatFastKernel<3>), grid, block, lds, stream, L); break;
             case 4: hipLaunchKernelGGL((beamFlatFastKernel<4>), grid, block, lds, stream, L); break;
+            case 5: hipLaunchKernelGGL((beamFlatFastKernel<5>), grid, block, lds, stream, L); break;
+            case 6: hipLaunchKernelGGL((beamFlatFastKernel<6>), grid, block, lds, stream, L); break;
+            case 7: hipLaunchKernelGGL((beamFlatFastKernel<7>), grid, block, lds, stream, L); break;
+            case 8: hipLaunchKernelGGL((beamFlatFastKernel<8>), grid, block, lds, stream, L); break;
             default: return hipErrorInvalidValue;
         }
         return hipGetLastError();

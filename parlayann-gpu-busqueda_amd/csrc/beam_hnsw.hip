@@ -137,8 +137,12 @@ __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* fi
                 v[r] = *reinterpret_cast<const int4*>(L.vec + (size_t)pend[b * 64 + i] * kVecBytes + sub * 16);
         }
         int part8[8];
+        const int normMask = (L.normsInRow && sub == 7) ? -1 : 0;
 #pragma unroll
-        for (int r = 0; r < 8; r++) part8[r] = slicePartial(v[r], qv);
+        for (int r = 0; r < 8; r++) {
+            v[r].w &= ~normMask;
+            part8[r] = slicePartial(v[r], qv);
+        }
         const int tot = transposeReduce8(part8, lane);
         // lane l holds candidate (l & 7) * 8 + (l >> 3); the map is its own inverse
         const int mine = __shfl(tot, (lane & 7) * 8 + (lane >> 3), 64);
@@ -219,7 +223,10 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
     const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t* inw = reinterpret_cast<uint32_t*>(lds + L.lds.inw);
     uint32_t* ldsFilt = reinterpret_cast<uint32_t*>(lds + L.lds.filt);
-    const int4 qv = *reinterpret_cast<const int4*>(L.queries + (size_t)q * kVecBytes + sub * 16);
+    // rows may hold |p|^2 in bytes 124..127 (lane 7's .w): masked out of the products
+    const int normMask = (L.normsInRow && sub == 7) ? -1 : 0;
+    int4 qv = *reinterpret_cast<const int4*>(L.queries + (size_t)q * kVecBytes + sub * 16);
+    qv.w &= ~normMask;
     const int qq = groupSum8(sliceNorm(qv));
     const int64_t lim0 = (visit > 0 && visit < (int64_t)L.n) ? visit : (int64_t)L.n;
     const int bits0 = hnswFilterBits(Ls);
@@ -266,7 +273,7 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
             }
             if (globalFilter) __builtin_amdgcn_s_waitcnt(0);
             __builtin_amdgcn_wave_barrier();
-            const int d = readlane(groupDistance(L.vec, pe, sub, qv, qq), 0);
+            const int d = readlane(groupDistance(L.vec, pe, sub, qv, qq, normMask), 0);
             const uint64_t key = ((uint64_t)(uint32_t)d << 32) | pe;
             if (!cand.hasDist((uint32_t)d, lane))
                 cand.insertAt(cand.countLess((uint64_t)(uint32_t)d << 32, lane), key, lane);

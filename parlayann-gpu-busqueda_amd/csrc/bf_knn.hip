@@ -63,20 +63,23 @@ __device__ __forceinline__ void bitonicSort(uint64_t (&key)[C], int lane) {
     }
 }
 
-__global__ __launch_bounds__(256) void rowNormsKernel(const int8_t* rows, uint32_t n, int32_t* out) {
+__global__ __launch_bounds__(256) void rowNormsKernel(int8_t* rows, uint32_t n, int32_t* out,
+                                                     bool embed) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int4* r = reinterpret_cast<const int4*>(rows + (size_t)i * kVecBytes);
+    int4* r = reinterpret_cast<int4*>(rows + (size_t)i * kVecBytes);
     int s = 0;
 #pragma unroll
     for (int j = 0; j < kVecBytes / 16; j++) {
-        const int4 v = r[j];
+        int4 v = r[j];
+        if (embed && j == kVecBytes / 16 - 1) v.w = 0;  // the norm's own bytes
         s = __builtin_amdgcn_sdot4(v.x, v.x, s, false);
         s = __builtin_amdgcn_sdot4(v.y, v.y, s, false);
         s = __builtin_amdgcn_sdot4(v.z, v.z, s, false);
         s = __builtin_amdgcn_sdot4(v.w, v.w, s, false);
     }
     out[i] = s;
+    if (embed) reinterpret_cast<int32_t*>(rows + (size_t)i * kVecBytes)[kVecBytes / 4 - 1] = s;
 }
 
 struct BfLds {
@@ -117,7 +120,8 @@ __device__ void mergeRow(BfLds& S, int r, uint32_t k, int lane) {
 __global__ __launch_bounds__(64) void bfKnnKernel(const int8_t* base, const int32_t* bnorm,
                                                  uint32_t n, const int8_t* queries,
                                                  const int32_t* qnorm, uint32_t nq, uint32_t k,
-                                                 uint32_t* outIds, int32_t* outDists) {
+                                                 bool normsInRow, uint32_t* outIds,
+                                                 int32_t* outDists) {
     __shared__ BfLds S;
     const int lane = laneId();
     const uint32_t q0 = blockIdx.x * kQ;
@@ -134,6 +138,8 @@ __global__ __launch_bounds__(64) void bfKnnKernel(const int8_t* base, const int3
         const int8_t* qr = queries + (size_t)(q0 + col) * kVecBytes;
         a0 = *reinterpret_cast<const v4i*>(qr + 16 * grp);
         a1 = *reinterpret_cast<const v4i*>(qr + 64 + 16 * grp);
+        // bytes 124..127 hold norms: zero the query's so q.p ignores them
+        if (normsInRow && grp == 3) a1.w = 0;
     }
     // output rows held by this lane: 4*grp + r
     int qn[4];
@@ -218,19 +224,20 @@ hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launchRowNorms(const int8_t* rows, uint32_t n, int32_t* out, hipStream_t s) {
+hipError_t launchRowNorms(int8_t* rows, uint32_t n, int32_t* out, bool embed, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(rowNormsKernel, dim3((n + 255) / 256), dim3(256), 0, s, rows, n, out);
+    hipLaunchKernelGGL(rowNormsKernel, dim3((n + 255) / 256), dim3(256), 0, s, rows, n, out, embed);
     return hipGetLastError();
 }
 
 hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_t n,
                                const int8_t* queries, const int32_t* qnorm, uint32_t nq,
-                               uint32_t k, uint32_t* outIds, int32_t* outDists, hipStream_t s) {
+                               uint32_t k, bool normsInRow, uint32_t* outIds, int32_t* outDists,
+                               hipStream_t s) {
     if (nq == 0) return hipSuccess;
     if (k == 0 || k > (uint32_t)kTopCap) return hipErrorInvalidValue;
     hipLaunchKernelGGL(bfKnnKernel, dim3((nq + kQ - 1) / kQ), dim3(64), 0, s, base, bnorm, n,
-                       queries, qnorm, nq, k, outIds, outDists);
+                       queries, qnorm, nq, k, normsInRow, outIds, outDists);
     return hipGetLastError();
 }
 

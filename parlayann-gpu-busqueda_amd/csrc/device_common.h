@@ -170,6 +170,17 @@ __device__ __forceinline__ int slicePartial(const int4& v, const int4& q) {
     return pp - 2 * qp;
 }
 
+// With rows that carry their squared norm in bytes 124..127 (the last slice, held by
+// lane 7 of each 8-lane group) and a query whose bytes 124..127 are zero, the partial of
+// one slice is (p.p from the row, in lane 7) - 2 q.p: normMask is -1 in lane 7, else 0.
+__device__ __forceinline__ int slicePartialN(const int4& v, const int4& q, int normMask) {
+    int qp = sdot4(v.x, q.x, 0);
+    qp = sdot4(v.y, q.y, qp);
+    qp = sdot4(v.z, q.z, qp);
+    qp = sdot4(v.w, q.w, qp);
+    return (v.w & normMask) - 2 * qp;
+}
+
 __device__ __forceinline__ int sliceNorm(const int4& q) {
     int s = sdot4(q.x, q.x, 0);
     s = sdot4(q.y, q.y, s);
@@ -178,9 +189,11 @@ __device__ __forceinline__ int sliceNorm(const int4& q) {
 }
 
 // Distance from the query slice (8 lanes x 16 B) to one row, in every lane of the group.
+// normMask: -1 in lane 7 when rows carry their norm in bytes 124..127 (q's are zero).
 __device__ __forceinline__ int groupDistance(const int8_t* rows, uint32_t id, int sub,
-                                             const int4& qv, int qq) {
-    const int4 v = *reinterpret_cast<const int4*>(rows + (size_t)id * kVecBytes + sub * 16);
+                                             const int4& qv, int qq, int normMask) {
+    int4 v = *reinterpret_cast<const int4*>(rows + (size_t)id * kVecBytes + sub * 16);
+    v.w &= ~normMask;
     return qq + groupSum8(slicePartial(v, qv));
 }
 

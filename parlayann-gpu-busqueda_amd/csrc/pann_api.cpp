@@ -207,7 +207,7 @@ int pann_vectors_create(const int8_t* host_rows, uint32_t n, uint32_t dim, pann_
     }
     int r = uploadRows(v->dev, host_rows, n, dim, nullptr, false);
     if (r == PANN_OK) {
-        e = launchRowNorms(v->dev, n, v->norms, nullptr);
+        e = launchRowNorms(v->dev, n, v->norms, dim <= (uint32_t)kNormInRowMaxDim, nullptr);
         if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess) r = setError(PANN_ERR_HIP, "row norms: %s", hipGetErrorString(e));
     }
@@ -232,7 +232,7 @@ int pann_vectors_copy_async(pann_vectors* v, const int8_t* host_rows, uint32_t n
     if (n > v->n) return setError(PANN_ERR_ARG, "%u rows do not fit in %u", n, v->n);
     const int r = uploadRows(v->dev, host_rows, n, v->dim, (hipStream_t)stream, true);
     if (r != PANN_OK) return r;
-    HIP_TRY(launchRowNorms(v->dev, n, v->norms, (hipStream_t)stream));
+    HIP_TRY(launchRowNorms(v->dev, n, v->norms, v->dim <= (uint32_t)kNormInRowMaxDim, (hipStream_t)stream));
     return PANN_OK;
 }
 
@@ -527,6 +527,7 @@ int pann_searcher_launch(pann_searcher* s, const pann_vectors* base, const pann_
     if (s->nq == 0) return PANN_OK;
     FlatLaunch L = s->L;
     L.base = base->dev;
+    L.normsInRow = base->dim <= (uint32_t)kNormInRowMaxDim;
     L.adj = g->adj;
     L.queries = queries->dev;
     L.qids = qids;
@@ -629,6 +630,7 @@ int pann_hnsw_searcher_launch(pann_searcher* s, const pann_vectors* queries, con
     if (s->nq == 0) return PANN_OK;
     HnswLaunch H = s->H;
     H.vec = h->nodeVec;
+    H.normsInRow = h->dim <= (uint32_t)kNormInRowMaxDim;
     H.adj0 = h->adj0;
     H.adjUp = h->adjUp;
     H.upStart = h->upStart;
@@ -671,7 +673,8 @@ int pann_bruteforce_knn(const pann_vectors* base, const pann_vectors* queries, u
     if (nq == 0) return PANN_OK;
     const hipStream_t st = (hipStream_t)stream;
     hipError_t e = launchBruteForceKnn(base->dev, base->norms, base->n, queries->dev, queries->norms,
-                                       nq, k, ids_dev, dists_dev, st);
+                                       nq, k, base->dim <= (uint32_t)kNormInRowMaxDim, ids_dev,
+                                       dists_dev, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return setError(PANN_ERR_HIP, "brute-force k-NN: %s", hipGetErrorString(e));
     return PANN_OK;

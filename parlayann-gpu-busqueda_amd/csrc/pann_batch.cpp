@@ -155,6 +155,7 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
     L.qids = nullptr;
     L.queries = b->queries;
     L.base = base->dev;
+    L.normsInRow = base->dim <= (uint32_t)kNormInRowMaxDim;
     L.adj = g->adj;
     L.outIds = b->out.ids;
     L.outDists = b->out.dists;
@@ -210,6 +211,7 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
     }
     HnswLaunch& H = b->H;
     H.vec = h->nodeVec;
+    H.normsInRow = h->dim <= (uint32_t)kNormInRowMaxDim;
     H.adj0 = h->adj0;
     H.adjUp = h->adjUp;
     H.upStart = h->upStart;

@@ -10,6 +10,8 @@ namespace pann {
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // padding of adjacency rows / missing neighbour
 constexpr int kVecBytes = 128;            // device row pitch of base and query vectors
 constexpr int kMaxDim = 128;              // widest vector the kernels accept
+// Rows of at most this many bytes carry their squared norm (int32) in bytes 124..127.
+constexpr int kNormInRowMaxDim = 124;
 constexpr int kWave = 64;
 constexpr int kTagSlots = 1024;           // per-wave byte table used to detect slot collisions
 constexpr int kFlagOverflow = 1;          // cuda/busqueda.cuh:14
@@ -112,6 +114,7 @@ __host__ __device__ inline uint32_t flatVisitedCap(int Ls) {
 
 struct FlatLaunch {
     const int8_t* base;        // n rows, kVecBytes pitch, zero padded
+    int32_t normsInRow;        // rows (and query rows) hold |p|^2 in bytes 124..127
     const uint32_t* adj;       // n x G, kEmpty padded
     const int8_t* queries;     // nq rows, kVecBytes pitch
     const uint32_t* qids;      // nullable: query index = row
@@ -129,7 +132,7 @@ struct FlatLaunch {
     FlatLds lds;
     int32_t fast;              // register-resident kernel (layout flatFastLdsLayout): 1, or
                                // 2 when base and adjacency are below 4 GiB; +2 with next-row
-                               // prefetch; 0: general
+                               // prefetch (the launcher adds 4 for normsInRow); 0: general
     uint32_t* gfilt;           // per-wave global filter regions
     uint32_t* queue;           // persistent mode: {next ticket, finished waves}; null: static
     const uint32_t* order;     // persistent mode: ticket -> query (null: identity)
@@ -171,6 +174,7 @@ __host__ __device__ inline HnswLds hnswLdsLayout(int maxch, uint32_t capInw, int
 
 struct HnswLaunch {
     const int8_t* vec;         // vectors in node order, kVecBytes pitch
+    int32_t normsInRow;        // rows hold |p|^2 in bytes 124..127
     const uint32_t* adj0;      // n x G0
     const uint32_t* adjUp;     // blocks x GA
     const uint32_t* upStart;   // n: first upper block of a node
@@ -208,12 +212,15 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
                           hipStream_t stream);
 hipError_t beamFlatOccupancy(int maxch, int fast, int wavesPerBlock, size_t ldsBytes,
                              int* blocksPerCU);
-hipError_t launchRowNorms(const int8_t* rows, uint32_t n, int32_t* out, hipStream_t s);
+// Squared norms of n rows into out; with embed, over bytes 0..123 only, also written
+// into bytes 124..127 of each row.
+hipError_t launchRowNorms(int8_t* rows, uint32_t n, int32_t* out, bool embed, hipStream_t s);
 // Per-section cycle counters of the flat search (profiling builds only).
 hipError_t profileCounters(uint64_t* out, int n, bool reset);
 hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_t n,
                                const int8_t* queries, const int32_t* qnorm, uint32_t nq,
-                               uint32_t k, uint32_t* outIds, int32_t* outDists, hipStream_t s);
+                               uint32_t k, bool normsInRow, uint32_t* outIds, int32_t* outDists,
+                               hipStream_t s);
 
 }  // namespace pann
