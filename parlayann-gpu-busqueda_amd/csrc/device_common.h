@@ -375,13 +375,22 @@ __device__ __forceinline__ bool seenFilterEnd(uint32_t* filt, const SeenProbe& s
         const int Lr = __ffsll((unsigned long long)cm) - 1;
         const uint32_t hL = (uint32_t)readlane((int)h, Lr);
         const uint64_t M = ballot(part && h == hL);
-        const bool inM = (M >> lane) & 1ull;
-        const uint64_t below = M & lanesBelow;
-        const uint64_t above = M & ~(lanesBelow | (1ull << lane));
-        const int src = (inM && below) ? (63 - __clzll((long long)below)) : lane;
-        const uint32_t x = (uint32_t)__shfl((int)a, src, 64);
-        if (inM && below) prev = x;
-        if (inM && above) last = false;
+        if (__popcll(M) == 2) {
+            // the usual group, two lanes: the upper one sees the lower one's id
+            const int lo = __ffsll((unsigned long long)M) - 1;
+            const int hi = 63 - __clzll((long long)M);
+            const uint32_t aLo = (uint32_t)readlane((int)a, lo);
+            if (lane == hi) prev = aLo;
+            if (lane == lo) last = false;
+        } else {
+            const bool inM = (M >> lane) & 1ull;
+            const uint64_t below = M & lanesBelow;
+            const uint64_t above = M & ~(lanesBelow | (1ull << lane));
+            const int src = (inM && below) ? (63 - __clzll((long long)below)) : lane;
+            const uint32_t x = (uint32_t)__shfl((int)a, src, 64);
+            if (inM && below) prev = x;
+            if (inM && above) last = false;
+        }
         cm &= ~M;
     }
     const bool seen = part && (prev == a);
