@@ -292,6 +292,89 @@ def calibrate_copy(nbytes, reps):
     print(json.dumps({"calibrate_copy_bytes": nbytes, "reps": reps}))
 
 
+def event_ms(launch):
+    """HIP-event duration of one launch on the null stream."""
+    from pann import _lib
+    import ctypes as C
+    L = _lib.lib()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    _lib.check(L.pann_event_create(C.byref(e0)))
+    _lib.check(L.pann_event_create(C.byref(e1)))
+    _lib.check(L.pann_event_record(e0.value, None))
+    launch()
+    _lib.check(L.pann_event_record(e1.value, None))
+    ms = C.c_float()
+    _lib.check(L.pann_event_elapsed_ms(e0.value, e1.value, C.byref(ms)))
+    L.pann_event_destroy(e0.value)
+    L.pann_event_destroy(e1.value)
+    return ms.value
+
+
+def sweep_graphs(args, base, queries, b, q, gt_ids, gt_df, k=10):
+    """The four graph families at full size over the 18 Ls values: GPU kernel QPS (HIP
+    events, plain launch over all queries), recall@10, distance computations per query,
+    and the oracle's QPS on all host cores over a bounded query sample."""
+    from oracle import oracle as O
+    from pann import builders, recall, search
+    cores = host_cores()
+    nthreads = min(16, cores)
+    sample = min(args.sweep_cpu_sample, len(queries))
+    qs = queries[:sample]
+    rows = []
+    for kind in args.sweep_graphs.split(","):
+        t = time.time()
+        if kind == "vamana":
+            host = load_or_build_graph(base, args, args.cache)
+        elif kind == "hcnng":
+            host = builders.build_hcnng(base, num_trees=30, leaf_size=1000, mst_deg=3, seed=7,
+                                        nthreads=nthreads)
+        elif kind == "pynndescent":
+            host = builders.build_pynndescent(base, K=40, num_trees=10, seed=7, nthreads=nthreads)
+        elif kind == "hnsw":
+            host = builders.build_hnsw(base, M=32, efc=128, seed=7, nthreads=nthreads)
+        else:
+            raise SystemExit(f"unknown graph kind {kind}")
+        build_s = time.time() - t
+        idx = search.HnswIndex(host, b) if kind == "hnsw" else search.FlatGraph(host)
+        log(f"[sweep] {kind}: built in {build_s:.1f}s")
+        for Ls in LS_SWEEP:
+            cfg = search.SearchConfig(Ls=Ls, k=k)
+            if kind == "hnsw":
+                s = search.HnswSearcher(idx, len(queries), cfg)
+                launch = lambda: s.launch(q, idx)
+            else:
+                s = search.Searcher(idx, len(queries), cfg)
+                launch = lambda: s.launch(b, q, idx)
+            launch()
+            search.synchronize()
+            r = s.download()
+            rec = recall.recall_at_k(gt_ids, gt_df, r.ids, k)
+            ms = min(event_ms(launch) for _ in range(3))
+            s.free()
+            params = O.make_params(sample, Ls, k)
+            run = (lambda: O.search_hnsw(base, host, qs, params, nthreads=cores)) if kind == "hnsw" \
+                else (lambda: O.beam_flat(base, host, qs, params, nthreads=cores))
+            run()
+            cpu = 0.0
+            for _ in range(2):
+                t = time.perf_counter()
+                run()
+                cpu = max(cpu, sample / (time.perf_counter() - t))
+            row = {"graph": kind, "Ls": Ls, "recall": round(rec, 5), "gpu_kernel_ms": round(ms, 4),
+                   "gpu_qps": round(len(queries) / ms * 1e3, 1), "cpu_qps": round(cpu, 1),
+                   "comps": round(float(r.comps.mean()), 1), "hops": round(float(r.hops.mean()), 1),
+                   "flagged": int(np.count_nonzero(r.flags & ~search.FLAG_SHORT)),
+                   "alg_gbs": round(float(100 * r.comps.astype(np.float64).sum()
+                                          + 4 * r.edges.astype(np.float64).sum()) / ms / 1e6, 1),
+                   "build_s": round(build_s, 1)}
+            rows.append(row)
+            log(f"[sweep] {kind} Ls={Ls}: recall {rec:.5f}, {row['gpu_qps']:,.0f} QPS GPU "
+                f"({ms:.3f} ms), {cpu:,.0f} QPS CPU, comps {row['comps']}")
+        idx.free()
+    return {"sweep": rows, "cpu_cores": cores, "cpu_sample_queries": sample,
+            "n": len(base), "nq": len(queries)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -309,6 +392,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the batched-vs-plain workload")
+    ap.add_argument("--sweep", action="store_true",
+                    help="instead of the headline line: the 18-Ls sweep over the four graph families")
+    ap.add_argument("--sweep-graphs", default="vamana,hcnng,pynndescent,hnsw")
+    ap.add_argument("--sweep-cpu-sample", type=int, default=2000)
     ap.add_argument("--no-tune", action="store_true",
                     help="time the batched form in its default shape instead of the autotuned one")
     ap.add_argument("--mixed-nq", type=int, default=65_536)
@@ -344,6 +431,9 @@ def main():
     gt_ids, gt_d = search.bruteforce_knn(b, q, 100)
     log(f"[bench] GPU ground truth top-100 in {time.time() - t:.2f}s")
     gt_df = gt_d.astype(np.float32)
+    if args.sweep:
+        print(json.dumps(sweep_graphs(args, base, queries, b, q, gt_ids, gt_df)))
+        return
 
     sweep = []
     chosen = None

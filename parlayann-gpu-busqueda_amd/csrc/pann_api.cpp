@@ -552,7 +552,7 @@ int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_searc
     const int maxch = G <= 64 ? 1 : G <= 128 ? 2 : 4;
     const int bits0 = hnswFilterBits(Ls);
     const bool ldsFilter = bits0 <= kFilterBitsLds;
-    const uint32_t capInw = hnswInwCap(Ls);
+    const uint32_t capInw = hnswInwCapFit(Ls, maxch, ldsFilter ? bits0 : 0, kMaxLdsPerBlock);
     const HnswLds lay = hnswLdsLayout(maxch, capInw, ldsFilter ? bits0 : 0);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "Ls=%d needs %u bytes of LDS per query", Ls, lay.total);

@@ -191,7 +191,8 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
     b->maxch = G <= 64 ? 1 : G <= 128 ? 2 : 4;
     const int fbMax = hnswFilterBits(LsMax);
     const int fbLds = std::max(2, std::min(cfg->lds_filter_bits ? cfg->lds_filter_bits : kFilterBitsLds, fbMax));
-    const uint32_t capInw = cfg->visited_slots ? cfg->visited_slots : hnswInwCap(LsMax);
+    const uint32_t capInw = cfg->visited_slots ? cfg->visited_slots
+                                               : hnswInwCapFit(LsMax, b->maxch, fbLds, kMaxLdsPerBlock);
     if (capInw < 256u * b->maxch) {
         delete b;
         return setError(PANN_ERR_CAPACITY, "visited_slots %u too small for rows of %u", capInw, G);

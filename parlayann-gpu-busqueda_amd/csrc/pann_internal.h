@@ -172,6 +172,14 @@ __host__ __device__ inline HnswLds hnswLdsLayout(int maxch, uint32_t capInw, int
     return L;
 }
 
+// Default is_inw capacity: hnswInwCap(Ls), halved until the per-wavefront layout fits
+// ldsLimit bytes (ef around 500); a query that would fill the table is flagged OVERFLOW.
+__host__ __device__ inline uint32_t hnswInwCapFit(int Ls, int maxch, int fbitsLds, uint32_t ldsLimit) {
+    uint32_t cap = hnswInwCap(Ls);
+    while (cap > 256u * (uint32_t)maxch && hnswLdsLayout(maxch, cap, fbitsLds).total > ldsLimit) cap >>= 1;
+    return cap;
+}
+
 struct HnswLaunch {
     const int8_t* vec;         // vectors in node order, kVecBytes pitch
     int32_t normsInRow;        // rows hold |p|^2 in bytes 124..127
