@@ -181,7 +181,7 @@ typedef struct pann_batch_config {
     int32_t max_Ls;          /* largest per-query Ls */
     int32_t max_k;           /* output stride */
     int32_t waves_per_block; /* 1, 2 or 4 (0: the most that fit 64 KB of LDS) */
-    int32_t lds_filter_bits; /* seen filters up to 2^bits slots live in LDS, 10..12 (0: 12) */
+    int32_t lds_filter_bits; /* seen filters up to 2^bits slots live in LDS, 2..12 (0: 10) */
     uint32_t visited_slots;  /* LDS visited / is_inw hash slots, power of two >= 64 (0: derived) */
     uint32_t prefetch;       /* next-row prefetch depth, 0 or 1: while a hop merges, load the
                                 row of the predicted next node (flat graphs, max_Ls <= 64) */
