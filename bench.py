@@ -375,6 +375,62 @@ def sweep_graphs(args, base, queries, b, q, gt_ids, gt_df, k=10):
             "n": len(base), "nq": len(queries)}
 
 
+def hnsw_large(args):
+    """HNSW at 10M x 128 int8 (GMM with 4,096 components), M=32, efC=200, mL=1/ln 32:
+    100,000 queries at ef in {16, 64, 256}, k=10.  Throughput and recall@10 on the full
+    query set; bit-exact check (ids, distances, counts, flags) against the oracle on the
+    first `--w4-check` queries, whose oracle time gives the CPU QPS."""
+    from oracle import oracle as O
+    from pann import builders, datagen, recall, search
+    n, nq, dim = args.w4_n, args.w4_nq, 128
+    cores = host_cores()
+    t = time.time()
+    base, queries = datagen.gmm_dataset(n, nq, dim=dim, ncomp=args.w4_ncomp)
+    log(f"[w4] data {base.shape} + {queries.shape} in {time.time() - t:.1f}s")
+    os.environ["PANN_BUILD_PROGRESS"] = "1"
+    t = time.time()
+    h = builders.build_hnsw(base, M=32, efc=200, seed=7, nthreads=min(16, cores))
+    build_s = time.time() - t
+    log(f"[w4] HNSW built in {build_s:.1f}s, top level {int(h.level.max())}")
+    b, q = search.Vectors(base), search.Vectors(queries)
+    idx = search.HnswIndex(h, b)
+    t = time.time()
+    gt_ids, gt_d = search.bruteforce_knn(b, q, 100)
+    gt_df = gt_d.astype(np.float32)
+    log(f"[w4] GPU ground truth top-100 in {time.time() - t:.2f}s")
+    sub = min(args.w4_check, nq)
+    rows = []
+    for ef in (16, 64, 256):
+        s = search.HnswSearcher(idx, nq, search.SearchConfig(Ls=ef, k=10))
+        s.launch(q, idx)
+        search.synchronize()
+        r = s.download()
+        ms = min(event_ms(lambda: s.launch(q, idx)) for _ in range(3))
+        s.free()
+        rec = recall.recall_at_k(gt_ids, gt_df, r.ids, 10)
+        params = O.make_params(sub, ef, 10)
+        O.search_hnsw(base, h, queries[:sub], params, nthreads=cores)
+        t = time.perf_counter()
+        o = O.search_hnsw(base, h, queries[:sub], params, nthreads=cores)
+        cpu_qps = sub / (time.perf_counter() - t)
+        same = ((r.ids[:sub] == o.ids[:, :10]).all(axis=1) & (r.dists[:sub] == o.dists[:, :10]).all(axis=1)
+                & (r.comps[:sub] == o.comps) & (r.flags[:sub] == o.flags) & (r.edges[:sub] == o.edges)
+                & (r.hops[:sub] == o.hops))
+        row = {"ef": ef, "recall": round(rec, 5), "gpu_kernel_ms": round(ms, 3),
+               "gpu_qps": round(nq / ms * 1e3, 1), "cpu_qps": round(cpu_qps, 1),
+               "comps": round(float(r.comps.mean()), 1), "hops": round(float(r.hops.mean()), 1),
+               "alg_gbs": round(float(dim * r.comps.astype(np.float64).sum()
+                                      + 4 * r.edges.astype(np.float64).sum()) / ms / 1e6, 1),
+               "checked": sub, "identical_to_oracle": int(same.sum()),
+               "flagged": int(np.count_nonzero(r.flags & ~search.FLAG_SHORT))}
+        rows.append(row)
+        log(f"[w4] ef={ef}: recall {rec:.5f}, {row['gpu_qps']:,.0f} QPS GPU ({ms:.3f} ms), "
+            f"{cpu_qps:,.0f} QPS CPU, {row['identical_to_oracle']}/{sub} identical to the oracle")
+    idx.free(), q.free(), b.free()
+    return {"workload": f"HNSW {n}x{dim} int8 GMM({args.w4_ncomp}), M=32 efC=200, {nq} queries, k=10",
+            "build_s": round(build_s, 1), "cpu_cores": cores, "rows": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,6 +452,12 @@ def main():
                     help="instead of the headline line: the 18-Ls sweep over the four graph families")
     ap.add_argument("--sweep-graphs", default="vamana,hcnng,pynndescent,hnsw")
     ap.add_argument("--sweep-cpu-sample", type=int, default=2000)
+    ap.add_argument("--hnsw-large", action="store_true",
+                    help="instead of the headline line: HNSW at 10M x 128 (ef 16/64/256)")
+    ap.add_argument("--w4-n", type=int, default=10_000_000)
+    ap.add_argument("--w4-nq", type=int, default=100_000)
+    ap.add_argument("--w4-ncomp", type=int, default=4096)
+    ap.add_argument("--w4-check", type=int, default=5000)
     ap.add_argument("--no-tune", action="store_true",
                     help="time the batched form in its default shape instead of the autotuned one")
     ap.add_argument("--mixed-nq", type=int, default=65_536)
@@ -420,6 +482,9 @@ def main():
 
     if search.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if args.hnsw_large:
+        print(json.dumps(hnsw_large(args)))
+        return
     k = 10
     t = time.time()
     base, queries = datagen.gmm_dataset(args.n, args.nq, dim=100, ncomp=args.ncomp)

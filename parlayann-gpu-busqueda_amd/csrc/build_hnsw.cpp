@@ -18,6 +18,8 @@
 
 #include "../../include/pann_tools.h"
 #include "tools_common.h"
+#include <cstdlib>
+#include <chrono>
 
 namespace pann {
 namespace tools {
@@ -224,11 +226,22 @@ extern "C" int pann_build_hnsw(const int8_t* base, uint32_t n, uint32_t dim, uin
     const uint32_t maxBatch = std::max<uint32_t>(1, (uint32_t)(0.02 * n));
     size_t count = 0;
     uint32_t bsize = 1;
+    // PANN_BUILD_PROGRESS=1: a line on stderr at every tenth of the points inserted
+    const char* pe = getenv("PANN_BUILD_PROGRESS");
+    const bool progress = pe && pe[0] == '1';
+    const auto t0 = std::chrono::steady_clock::now();
+    size_t nextReport = order.size() / 10;
     while (count < order.size()) {
         const size_t size = std::min<size_t>(bsize, order.size() - count);
         h.insertBatch(order.data() + count, size, searchers);
         count += size;
         bsize = std::min(maxBatch, bsize * 2);
+        if (progress && count >= nextReport) {
+            const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            fprintf(stderr, "[hnsw build] %zu of %zu points inserted, %.1f s\n", count, order.size(), s);
+            fflush(stderr);
+            nextReport += order.size() / 10;
+        }
     }
     return h.write(out_path, ml);
 }
