@@ -259,12 +259,17 @@ def batched_vs_plain(b, g, rows, params, steps, tune_sample=8192):
     }
 
 
-def serving(b, g, queries, gt_ids, gt_d, rates, duration):
-    """Poisson arrivals on the resident index: requests of 1..4096 queries, per-query Ls
-    log-uniform in [10, 300], k in {10, 100}; p50/p99 request latency (arrival to results
-    in host memory), sustained QPS and recall@10 of the served queries."""
+def serving(b, g, queries, gt_ids, gt_d, rates, duration, hnsw=None):
+    """Poisson arrivals on the resident index (flat graph g, or the HNSW index): requests
+    of 1..4096 queries, per-query Ls log-uniform in [10, 300], k in {10, 100}; p50/p99
+    request latency (arrival to results in host memory), sustained QPS and recall@10 of
+    the served queries."""
     from pann import recall, search
-    srv = search.Server(slots=65536, max_batch=16384, streams=3, max_Ls=300, max_k=100, base=b, graph=g)
+    if hnsw is not None:
+        srv = search.Server(slots=65536, max_batch=16384, streams=3, max_Ls=300, max_k=100, hnsw=hnsw)
+    else:
+        srv = search.Server(slots=65536, max_batch=16384, streams=3, max_Ls=300, max_k=100, base=b,
+                            graph=g)
     out = []
     srv.run_poisson(queries, 50000, 0.3, seed=99)  # warm-up
     for i, rate in enumerate(rates):
@@ -275,7 +280,7 @@ def serving(b, g, queries, gt_ids, gt_d, rates, duration):
                     "requests": int(st["requests"]), "queries": int(st["queries"]),
                     "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                     "max_ms": round(st["max_ms"], 3), "recall@10": round(rec, 5) if rec is not None else None})
-        log(f"[bench] serving {rate:,.0f} QPS offered: sustained {out[-1]['sustained_qps']:,.0f}, "
+        log(f"[bench] serving{' HNSW' if hnsw is not None else ''} {rate:,.0f} QPS offered: sustained {out[-1]['sustained_qps']:,.0f}, "
             f"p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms, recall@10 {out[-1]['recall@10']}")
     srv.free()
     return out
@@ -464,6 +469,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-rates", default="200000,1000000,2000000")
     ap.add_argument("--serving-seconds", type=float, default=2.0)
+    ap.add_argument("--no-serving-hnsw", action="store_true")
+    ap.add_argument("--serving-hnsw-rates", default="200000,1000000")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run the copy kernel over this many bytes (counter calibration)")
@@ -555,6 +562,17 @@ def main():
     if not args.no_serving:
         serve = serving(b, g, queries, gt_ids, gt_df, [float(x) for x in args.serving_rates.split(",")],
                         args.serving_seconds)
+    serve_hnsw = None
+    if not args.no_serving and not args.no_serving_hnsw:
+        from pann import builders
+        t = time.time()
+        hg = builders.build_hnsw(base, M=32, efc=128, seed=7, nthreads=min(16, host_cores()))
+        log(f"[bench] HNSW M=32 efC=128 built in {time.time() - t:.1f}s for the serving loop")
+        hidx = search.HnswIndex(hg, b)
+        serve_hnsw = serving(b, None, queries, gt_ids, gt_df,
+                             [float(x) for x in args.serving_hnsw_rates.split(",")], args.serving_seconds,
+                             hnsw=hidx)
+        hidx.free()
 
     cpu = None
     if not args.no_cpu:
@@ -600,6 +618,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_vs_plain": mixed,
         "serving": serve,
+        "serving_hnsw": serve_hnsw,
     }
     s.free(), g.free(), q.free(), b.free()
     print(json.dumps(out), flush=True)
