@@ -368,11 +368,12 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
 }
 
 // ---------------------------------------------------------------------------------------
-// Register-resident form for Ls <= 64 and rows of at most 64 entries.  Same semantics as
-// searchOne; lane i holds frontier entry i as one tagged 64-bit key
-//     d (24 bits) | id (32 bits) | tag (8 bits)
-// (an exact int8 squared distance is below 128 * 255^2 < 2^23).  Tag bit 7 is 0 for a
-// frontier copy, whose bits 1..6 hold its position (equal keys keep their order through
+// Register-resident form for Ls <= 64 (128 with two keys per lane) and rows of at most
+// 64 entries.  Same semantics as searchOne; lane i holds frontier entry i as one tagged
+// 64-bit key
+//     d (23 bits) | id (32 bits) | tag (9 bits)
+// (an exact int8 squared distance is below 128 * 255^2 < 2^23).  Tag bit 8 is 0 for a
+// frontier copy, whose bits 1..7 hold its position (equal keys keep their order through
 // the merge) and bit 0 its visited flag; it is 1 for a candidate, whose bits 0..6 hold
 // how often its id was expanded (saturated at 127; a row has at most 64 copies).
 //   * candidate vectors are loaded straight from the row layout (entry e = lane e) as
@@ -382,14 +383,17 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
 //     copies come first; candidate copy rep (0-based) enters iff rep >= cF, visited iff
 //     rep < expansions(id) -- the multiset union of the reference's search.
 constexpr uint64_t kNoKey = ~0ull;
+constexpr int kTagBits = 9;
+constexpr uint32_t kCand = 0x100u;        // tag: candidate copy
+constexpr uint64_t kTagMask = 0x1FFull;
 constexpr int kInsertMax = 16;  // up to this many candidates are inserted one by one
 
 __device__ __forceinline__ uint64_t tagKey(int d, uint32_t id, uint32_t tag) {
-    return ((uint64_t)(uint32_t)d << 40) | ((uint64_t)id << 8) | tag;
+    return ((uint64_t)(uint32_t)d << 41) | ((uint64_t)id << kTagBits) | tag;
 }
-__device__ __forceinline__ int keyDist(uint64_t k) { return (int)((uint32_t)(k >> 32) >> 8); }
+__device__ __forceinline__ int keyDist(uint64_t k) { return (int)((uint32_t)(k >> 32) >> 9); }
 __device__ __forceinline__ uint32_t keyId(uint64_t k) {
-    return ((uint32_t)(k >> 32) << 24) | ((uint32_t)k >> 8);
+    return ((uint32_t)(k >> 32) << 23) | ((uint32_t)k >> 9);
 }
 
 // Row `i` of a table at byte offset i * pitch + extra: a 32-bit offset from a scalar base
@@ -453,9 +457,9 @@ __device__ __forceinline__ UnionMarks unionMarks(uint64_t& x, uint64_t prev, int
                                                  int carryRun, int carryC, int lane,
                                                  uint64_t upTo) {
     const bool valid = x != kNoKey;
-    const bool isC = valid && (x & 0x80u);
-    const bool same = valid && ((x ^ prev) >> 8) == 0;
-    const bool prevC = (prev & 0x80u) != 0;
+    const bool isC = valid && (x & kCand);
+    const bool same = valid && ((x ^ prev) >> kTagBits) == 0;
+    const bool prevC = (prev & kCand) != 0;
     int run = lastAtOrBelow(ballot(valid && !same), upTo);
     run = run < 0 ? carryRun : base + run;
     int fc = lastAtOrBelow(ballot(isC && !(same && prevC)), upTo);
@@ -568,7 +572,7 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
         P.count(kProfHops);
 
         const int cutoff = (fsize < Ls) ? 0x7FFFFFFF
-                                        : (readlane((int)(uint32_t)(F >> 32), fsize - 1) >> 8);
+                                        : (int)((uint32_t)readlane((int)(uint32_t)(F >> 32), fsize - 1) >> 9);
 
         // seen filter, first half: slots and tags
         edges += __popcll(ballot(a != kEmpty));
@@ -628,7 +632,7 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
         // p.p is recomputed from the row: a stored norm would cost one more random
         // cache line per candidate
         const int d = qq + transposeReduce8(part8, lane);  // distance of entry `lane`
-        const uint64_t C = (keep && d < cutoff) ? tagKey(d, a, 0x80u | cnt) : kNoKey;
+        const uint64_t C = (keep && d < cutoff) ? tagKey(d, a, kCand | cnt) : kNoKey;
         P.mark(kProfDist);
 
         // Prefetch the row of the likely next node: the better (by distance) of the next
@@ -676,7 +680,7 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
                 const int j = __ffsll((unsigned long long)mm) - 1;
                 mm &= mm - 1;
                 const uint64_t x = readlane64(C, j);
-                const uint64_t xmin = x & ~0xFFull, xmax = x | 0xFFull;
+                const uint64_t xmin = x & ~kTagMask, xmax = x | kTagMask;
                 const int pos = __popcll(ballot(F <= xmax));
                 if (pos >= Ls) continue;
                 const int lower = __popcll(ballot(F < xmin));
@@ -686,10 +690,10 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
                 // entering copy keeps "expansions - rep" in its tag: visited iff positive.
                 uint64_t xe = x;
                 if (pos != lower) {
-                    const int cF = __popcll(ballot(F >= xmin && F <= xmax && !(F & 0x80u)));
+                    const int cF = __popcll(ballot(F >= xmin && F <= xmax && !(F & kCand)));
                     const int rep = __popcll(ballot(C == x) & ((1ull << j) - 1));
                     if (rep < cF) continue;
-                    xe = xmin | 0x80u | (uint32_t)max(0, (int)(x & 0x7Fu) - rep);
+                    xe = xmin | kCand | (uint32_t)max(0, (int)(x & 0x7Fu) - rep);
                 }
                 // shift the tail up one lane, then place the key (lane 0 never takes the
                 // shifted value: pos >= 0)
@@ -701,8 +705,8 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
                 changed = true;
             }
             if (changed) {
-                const uint32_t vis = (F & 0x80u) ? ((F & 0x7Fu) ? 1u : 0u) : (uint32_t)(F & 1u);
-                F = lane < fs ? ((F & ~0xFFull) | ((uint64_t)lane << 1) | vis) : kNoKey;
+                const uint32_t vis = (F & kCand) ? ((F & 0x7Fu) ? 1u : 0u) : (uint32_t)(F & 1u);
+                F = lane < fs ? ((F & ~kTagMask) | ((uint64_t)lane << 1) | vis) : kNoKey;
                 fsize = fs;
             }
             P.mark(kProfMerge);
@@ -734,7 +738,7 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
                 lo = lane < newSize ? scratch[lane] : kNoKey;
                 __builtin_amdgcn_wave_barrier();
             }
-            F = lane < newSize ? ((lo & ~0xFFull) | ((uint64_t)lane << 1) | (lo & 1u)) : kNoKey;
+            F = lane < newSize ? ((lo & ~kTagMask) | ((uint64_t)lane << 1) | (lo & 1u)) : kNoKey;
             fsize = newSize;
             changed = true;
             P.mark(kProfMerge);
@@ -743,7 +747,7 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
         // cut: upper_bound(F, (0, (float)(cut * (double)F[k].d))); a frontier that did
         // not change since the last cut is already cut
         if (changed && c.k > 0 && fsize > c.k) {
-            const int dk = readlane((int)(uint32_t)(F >> 32), c.k) >> 8;
+            const int dk = (int)((uint32_t)readlane((int)(uint32_t)(F >> 32), c.k) >> 9);
             const float t = (float)(c.cut * (double)dk);
             const float df = (float)keyDist(F);
             const bool pred = lane < fsize && ((t < df) || (t == df && keyId(F) > 0u));
@@ -771,6 +775,304 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
         if (j < fsize) {
             oi[j] = keyId(F);
             od[j] = keyDist(F);
+        } else {
+            oi[j] = kEmpty;
+            od[j] = 0x7FFFFFFF;
+        }
+    }
+    if (lane == 0) {
+        if (L.outComps) L.outComps[q] = comps;
+        if (L.outFlags) L.outFlags[q] = flags;
+        if (L.outEdges) L.outEdges[q] = edges;
+        if (L.outHops) L.outHops[q] = hops;
+    }
+    __builtin_amdgcn_wave_barrier();
+    P.mark(kProfOutput);
+}
+
+// lane i <- lane 63 - i (row mirror, then the rows in reverse order)
+__device__ __forceinline__ uint64_t reverseWave(uint64_t x, int lane) {
+    const uint64_t m = ((uint64_t)dppMov<0x140>((uint32_t)(x >> 32)) << 32) | dppMov<0x140>((uint32_t)x);
+    return xorLane64(xorLane64(m, 32, lane), 16, lane);
+}
+
+// Key at frontier position p (uniform) of a two-register frontier.
+__device__ __forceinline__ uint64_t frontierKey(uint64_t F0, uint64_t F1, int p) {
+    return p < 64 ? readlane64(F0, p) : readlane64(F1, p - 64);
+}
+
+// The register-resident search for 64 < Ls <= 128: positions 0..63 in F0, 64..127 in F1,
+// the seen filter (2^11 or 2^12 slots) in the wavefront's HBM region.  Same steps as
+// searchFast; the union of more than kInsertMax candidates merges the descending
+// candidates with F0, then the upper half of that with F1: every element of the first
+// half is <= every element of F1, so the two merges give the sorted union.
+template <bool NARROW, bool NORMS>
+__device__ void searchFast2(const FlatLaunch& L, unsigned char* lds, uint32_t q, const QueryCfg& c,
+                            uint32_t* filt, Prof& P) {
+    const int lane = laneId();
+    const int grp = lane >> 3, sub = lane & 7;
+    const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t upTo = lanesBelow | (1ull << lane);
+
+    uint64_t* scratch = reinterpret_cast<uint64_t*>(lds + L.lds.kkey);  // 128 keys
+    unsigned char* tag = lds + L.lds.tag;
+    uint32_t* vkey = reinterpret_cast<uint32_t*>(lds + L.lds.vkey);
+    unsigned char* vcnt = lds + L.lds.vcnt;
+
+    const int Ls = c.Ls;
+    const int fbits = flatFilterBits(Ls);
+    const uint32_t fmask = (1u << fbits) - 1;
+    uint32_t capV = flatVisitedCap(Ls);
+    if (capV > L.capV) capV = L.capV;
+    const uint32_t logCap = (uint32_t)ceilLog2(capV);
+    const uint32_t vmask = capV - 1;
+    const uint32_t win = capV < 64 ? capV : 64;
+
+    const int normMask = (NORMS && sub == 7) ? -1 : 0;
+    int4 qv = *reinterpret_cast<const int4*>(L.queries + (size_t)q * kVecBytes + sub * 16);
+    qv.w &= ~normMask;
+    const int qq = groupSum8(sliceNorm(qv));
+
+    {
+        const uint4 e4 = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+        filtClear<true>(filt, fbits, lane);
+        uint4* v4 = reinterpret_cast<uint4*>(vkey);
+        for (uint32_t i = lane; i < capV / 4; i += 64) v4[i] = e4;
+    }
+
+    const int d0 = groupDistance(L.base, 0u, sub, qv, qq, normMask);
+    uint64_t F0 = lane == 0 ? tagKey(d0, 0u, 0u) : kNoKey;
+    uint64_t F1 = kNoKey;
+    int fsize = 1;
+    int u = 0;
+    int64_t visits = 0;
+    int comps = 1, edges = 0, hops = 0, flags = 0;
+    uint32_t distinct = 0;
+    const uint32_t G = L.G;
+    __builtin_amdgcn_wave_barrier();
+    P.mark(kProfInit);
+    P.count(kProfQueries);
+
+    while (u >= 0 && visits < c.limit) {
+        const uint32_t uid = keyId(frontierKey(F0, F1, u));
+        const uint32_t a =
+            (uint32_t)lane < G ? *rowAt<NARROW, uint32_t>(L.adj, uid, 4 * G, 4 * (uint32_t)lane) : kEmpty;
+
+        if (u < 64) {
+            if (lane == u) F0 |= 1u;
+        } else if (lane == u - 64) {
+            F1 |= 1u;
+        }
+        bool overflow = false;
+        {
+            const uint32_t h0 = visitedSlot(uid, logCap);
+            for (uint32_t off = 0; off < capV; off += win) {
+                const uint32_t s = (h0 + off + (uint32_t)lane) & vmask;
+                const bool act = (uint32_t)lane < win;
+                const uint32_t kk = act ? vkey[s] : 0u;
+                const uint64_t hit = ballot(act && kk == uid);
+                const uint64_t m = hit | ballot(act && kk == kEmpty);
+                if (m) {
+                    const int p = __ffsll((unsigned long long)m) - 1;
+                    if ((hit >> p) & 1ull) {
+                        if (lane == p) vcnt[s] = (unsigned char)(vcnt[s] + 1);
+                    } else if (distinct + 1 < capV) {
+                        if (lane == p) {
+                            vkey[s] = uid;
+                            vcnt[s] = 1;
+                        }
+                        distinct++;
+                    } else {
+                        overflow = true;
+                    }
+                    break;
+                }
+            }
+        }
+        if (overflow) {
+            flags |= kFlagOverflow;
+            break;
+        }
+        visits++;
+        hops++;
+        P.mark(kProfVisit);
+        P.count(kProfHops);
+
+        const int cutoff = (fsize < Ls) ? 0x7FFFFFFF : keyDist(frontierKey(F0, F1, fsize - 1));
+
+        edges += __popcll(ballot(a != kEmpty));
+        const bool part = (a != kEmpty) && (a != c.qid);
+        const SeenProbe sp = seenFilterBegin<true>(filt, tag, fmask, a, part, lane);
+        const bool spec = part && (sp.coll || sp.prev != a);
+
+        const uint32_t gmask = (uint32_t)(ballot(spec) >> (8 * grp)) & 0xFFu;
+        uint32_t ids[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) ids[r] = groupBroadcast8(a, r);
+        int4 v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            asm("" : "=v"(v[r].x), "=v"(v[r].y), "=v"(v[r].z), "=v"(v[r].w));
+            if ((gmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
+        }
+
+        uint32_t cnt = 0;
+        if (spec) {
+            uint32_t hh = visitedSlot(a, logCap);
+            while (true) {
+                const uint32_t kk = vkey[hh];
+                if (kk == a) {
+                    cnt = min(127u, (uint32_t)vcnt[hh]);
+                    break;
+                }
+                if (kk == kEmpty) break;
+                hh = (hh + 1) & vmask;
+            }
+        }
+
+        const bool keep = seenFilterEnd<true>(filt, sp, a, part, lane, lanesBelow);
+        const uint64_t keepM = ballot(keep);
+        comps += __popcll(keepM);
+        const uint64_t late = keepM & ~ballot(spec);
+        if (late) {
+            const uint32_t lmask = (uint32_t)(late >> (8 * grp)) & 0xFFu;
+#pragma unroll
+            for (int r = 0; r < 8; r++)
+                if ((lmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
+        }
+        P.mark(kProfFilter);
+
+        int part8[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+            part8[r] = NORMS ? slicePartialN(v[r], qv, normMask) : slicePartial(v[r], qv);
+        const int d = qq + transposeReduce8(part8, lane);
+        const uint64_t C = (keep && d < cutoff) ? tagKey(d, a, kCand | cnt) : kNoKey;
+        P.mark(kProfDist);
+
+        const uint64_t candM = ballot(C != kNoKey);
+        const int nc = __popcll(candM);
+        bool changed = false;
+        if (nc > 0 && nc <= kInsertMax) {
+            int fs = fsize;
+            uint64_t mm = candM;
+            while (mm) {
+                const int j = __ffsll((unsigned long long)mm) - 1;
+                mm &= mm - 1;
+                const uint64_t x = readlane64(C, j);
+                const uint64_t xmin = x & ~kTagMask, xmax = x | kTagMask;
+                const int pos = __popcll(ballot(F0 <= xmax)) + __popcll(ballot(F1 <= xmax));
+                if (pos >= Ls) continue;
+                const int lower = __popcll(ballot(F0 < xmin)) + __popcll(ballot(F1 < xmin));
+                uint64_t xe = x;
+                if (pos != lower) {
+                    const int cF = __popcll(ballot(F0 >= xmin && F0 <= xmax && !(F0 & kCand))) +
+                                   __popcll(ballot(F1 >= xmin && F1 <= xmax && !(F1 & kCand)));
+                    const int rep = __popcll(ballot(C == x) & ((1ull << j) - 1));
+                    if (rep < cF) continue;
+                    xe = xmin | kCand | (uint32_t)max(0, (int)(x & 0x7Fu) - rep);
+                }
+                // shift positions >= pos up by one (F0's lane 63 moves to F1's lane 0)
+                const uint64_t carry = readlane64(F0, 63);
+                uint64_t pv1 = ((uint64_t)dppMov<0x138>((uint32_t)(F1 >> 32)) << 32) |
+                               dppMov<0x138>((uint32_t)F1);
+                if (lane == 0) pv1 = carry;
+                if (pos < 64) {
+                    const uint64_t pv0 = ((uint64_t)dppMov<0x138>((uint32_t)(F0 >> 32)) << 32) |
+                                         dppMov<0x138>((uint32_t)F0);
+                    F1 = pv1;
+                    if (lane > pos) F0 = pv0;
+                    if (lane == pos) F0 = xe;
+                } else {
+                    const int p1 = pos - 64;
+                    if (lane > p1) F1 = pv1;
+                    if (lane == p1) F1 = xe;
+                }
+                fs = min(Ls, fs + 1);
+                changed = true;
+            }
+            if (changed) {
+                const uint32_t vis0 = (F0 & kCand) ? ((F0 & 0x7Fu) ? 1u : 0u) : (uint32_t)(F0 & 1u);
+                const uint32_t vis1 = (F1 & kCand) ? ((F1 & 0x7Fu) ? 1u : 0u) : (uint32_t)(F1 & 1u);
+                F0 = lane < fs ? ((F0 & ~kTagMask) | ((uint64_t)lane << 1) | vis0) : kNoKey;
+                F1 = lane + 64 < fs ? ((F1 & ~kTagMask) | ((uint64_t)(lane + 64) << 1) | vis1) : kNoKey;
+                fsize = fs;
+            }
+            P.mark(kProfMerge);
+        } else if (nc > kInsertMax) {
+            const uint64_t Cd = sortWave<true>(C, lane);
+            P.mark(kProfSort);
+            // F0 (ascending) with the candidates (descending), then the upper half of that
+            // (reversed to descending) with F1
+            uint64_t lo1 = F0 < Cd ? F0 : Cd;
+            uint64_t hi1 = F0 < Cd ? Cd : F0;
+            lo1 = mergeHalf(lo1, lane);
+            hi1 = reverseWave(mergeHalf(hi1, lane), lane);
+            uint64_t lo2 = F1 < hi1 ? F1 : hi1;
+            uint64_t hi2 = F1 < hi1 ? hi1 : F1;
+            lo2 = mergeHalf(lo2, lane);
+            const UnionMarks m1 = unionMarks(lo1, prevLane64(lo1, kNoKey, lane), 0, 0, 0, lane, upTo);
+            const UnionMarks m2 = unionMarks(lo2, prevLane64(lo2, readlane64(lo1, 63), lane), 64,
+                                             m1.lastRun, m1.lastC, lane, upTo);
+            const int v1 = __popcll(ballot(lo1 != kNoKey)), v2 = __popcll(ballot(lo2 != kNoKey));
+            int newSize;
+            if ((m1.drop | m2.drop) == 0) {
+                newSize = min(Ls, v1 + v2);
+                F0 = lo1;
+                F1 = lo2;
+            } else {
+                hi2 = mergeHalf(hi2, lane);
+                const UnionMarks m3 = unionMarks(hi2, prevLane64(hi2, readlane64(lo2, 63), lane), 128,
+                                                 m2.lastRun, m2.lastC, lane, upTo);
+                const int v3 = __popcll(ballot(hi2 != kNoKey));
+                const int n1 = __popcll(m1.drop), n2 = __popcll(m2.drop);
+                const int d1 = lane - (int)__popcll(m1.drop & lanesBelow);
+                const int d2 = 64 + lane - n1 - (int)__popcll(m2.drop & lanesBelow);
+                const int d3 = 128 + lane - n1 - n2 - (int)__popcll(m3.drop & lanesBelow);
+                if (lo1 != kNoKey && !((m1.drop >> lane) & 1ull)) scratch[d1] = lo1;
+                if (lo2 != kNoKey && !((m2.drop >> lane) & 1ull) && d2 < 128) scratch[d2] = lo2;
+                if (hi2 != kNoKey && !((m3.drop >> lane) & 1ull) && d3 < 128) scratch[d3] = hi2;
+                __builtin_amdgcn_wave_barrier();
+                newSize = min(Ls, v1 + v2 + v3 - n1 - n2 - (int)__popcll(m3.drop));
+                F0 = lane < newSize ? scratch[lane] : kNoKey;
+                F1 = lane + 64 < newSize ? scratch[64 + lane] : kNoKey;
+                __builtin_amdgcn_wave_barrier();
+            }
+            F0 = lane < newSize ? ((F0 & ~kTagMask) | ((uint64_t)lane << 1) | (F0 & 1u)) : kNoKey;
+            F1 = lane + 64 < newSize ? ((F1 & ~kTagMask) | ((uint64_t)(lane + 64) << 1) | (F1 & 1u)) : kNoKey;
+            fsize = newSize;
+            changed = true;
+            P.mark(kProfMerge);
+        }
+
+        if (changed && c.k > 0 && fsize > c.k) {
+            const int dk = keyDist(frontierKey(F0, F1, c.k));
+            const float t = (float)(c.cut * (double)dk);
+            const float df0 = (float)keyDist(F0), df1 = (float)keyDist(F1);
+            const uint64_t m0 = ballot(lane < fsize && ((t < df0) || (t == df0 && keyId(F0) > 0u)));
+            const uint64_t m1 = ballot(lane + 64 < fsize && ((t < df1) || (t == df1 && keyId(F1) > 0u)));
+            if (m0 | m1) {
+                fsize = m0 ? __ffsll((unsigned long long)m0) - 1 : 64 + __ffsll((unsigned long long)m1) - 1;
+                if (lane >= fsize) F0 = kNoKey;
+                if (lane + 64 >= fsize) F1 = kNoKey;
+            }
+        }
+        P.mark(kProfCut);
+
+        const uint64_t nv0 = ballot(lane < fsize && !(F0 & 1u));
+        const uint64_t nv1 = ballot(lane + 64 < fsize && !(F1 & 1u));
+        u = nv0 ? __ffsll((unsigned long long)nv0) - 1 : nv1 ? 64 + __ffsll((unsigned long long)nv1) - 1 : -1;
+        P.mark(kProfNext);
+    }
+
+    if (fsize < c.k) flags |= kFlagShort;
+    uint32_t* oi = L.outIds + (size_t)q * L.outStride;
+    int32_t* od = L.outDists + (size_t)q * L.outStride;
+    for (int j = lane; j < c.k; j += 64) {
+        if (j < fsize) {
+            const uint64_t kv = j < 64 ? F0 : F1;
+            oi[j] = keyId(kv);
+            od[j] = keyDist(kv);
         } else {
             oi[j] = kEmpty;
             od[j] = 0x7FFFFFFF;
@@ -822,7 +1124,13 @@ __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
         }
         makeUniform(c);
         const int fb = flatFilterBits(c.Ls);
-        if constexpr (FAST != 0) {
+        if constexpr (FAST >= 9) {
+            if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride || fb > L.gbitsMax) {
+                if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
+            } else {
+                searchFast2<((FAST - 9) & 1) != 0, ((FAST - 9) & 2) != 0>(L, lds, q, c, gfilt, P);
+            }
+        } else if constexpr (FAST != 0) {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride) {
                 if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
             } else {
@@ -860,6 +1168,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     drainQueries<1, FAST>(L);
 }
 
+// The two-register form: its LDS (6-11 KB per wavefront) allows about four wavefronts
+// per SIMD, so it is held to 128 VGPRs.
+template <int FAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void beamFlatFast2Kernel(FlatLaunch L) {
+    drainQueries<1, FAST>(L);
+}
+
 }  // namespace
 
 #ifdef PANN_PROFILE
@@ -886,7 +1201,19 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
     const size_t lds = (size_t)L.ldsPerWave * wavesPerBlock;
     const dim3 grid(blocks), block(64 * wavesPerBlock);
     if (L.fast) {
-        if (maxch != 1 || L.LsMax > kFastLsMax) return hipErrorInvalidValue;
+        if (maxch != 1) return hipErrorInvalidValue;
+        if (L.fast >= 9) {
+            if (L.LsMax > kFast2LsMax || L.gfilt == nullptr) return hipErrorInvalidValue;
+            switch (L.fast + (L.normsInRow ? 2 : 0)) {
+                case 9: hipLaunchKernelGGL((beamFlatFast2Kernel<9>), grid, block, lds, stream, L); break;
+                case 10: hipLaunchKernelGGL((beamFlatFast2Kernel<10>), grid, block, lds, stream, L); break;
+                case 11: hipLaunchKernelGGL((beamFlatFast2Kernel<11>), grid, block, lds, stream, L); break;
+                case 12: hipLaunchKernelGGL((beamFlatFast2Kernel<12>), grid, block, lds, stream, L); break;
+                default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        }
+        if (L.LsMax > kFastLsMax) return hipErrorInvalidValue;
         const int mode = L.fast + (L.normsInRow ? 4 : 0);
         switch (mode) {
             case 1: hipLaunchKernelGGL((beamFlatFastKernel<1>), grid, block, lds, stream, L); break;
@@ -913,6 +1240,11 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
 hipError_t beamFlatOccupancy(int maxch, int fast, int wavesPerBlock, size_t ldsBytes,
                              int* blocksPerCU) {
     const int threads = 64 * wavesPerBlock;
+    if (fast >= 9) {
+        if (maxch != 1) return hipErrorInvalidValue;
+        return fast == 9 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFast2Kernel<9>, threads, ldsBytes)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFast2Kernel<10>, threads, ldsBytes);
+    }
     if (fast) {
         if (maxch != 1) return hipErrorInvalidValue;
         switch (fast) {

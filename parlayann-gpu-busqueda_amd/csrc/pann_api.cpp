@@ -450,10 +450,14 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
     const int maxch = g->G <= 64 ? 1 : g->G <= 128 ? 2 : 4;
     const int Ls = cfg->Ls;
     const int fbits = flatFilterBits(Ls);
-    const bool ldsFilter = fbits <= kFilterBitsLds;
     const uint32_t capV = flatVisitedCap(Ls);
-    const bool fast = maxch == 1 && Ls <= kFastLsMax;
-    const FlatLds lay = fast ? flatFastLdsLayout(capV) : flatLdsLayout(Ls, maxch, capV, ldsFilter ? fbits : 0);
+    const bool fast = maxch == 1 && Ls <= kFastLsMax;             // one frontier key per lane
+    const bool fast2 = maxch == 1 && !fast && Ls <= kFast2LsMax;  // two, filter in HBM
+    const bool ldsFilter = fbits <= kFilterBitsLds && !fast2;
+    const int fastKind = fast ? fastMode(g->n, g->G) : fast2 ? 8 + fastMode(g->n, g->G) : 0;
+    const FlatLds lay = fast ? flatFastLdsLayout(capV)
+                      : fast2 ? flatFast2LdsLayout(capV)
+                              : flatLdsLayout(Ls, maxch, capV, ldsFilter ? fbits : 0);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "Ls=%d needs %u bytes of LDS per query", Ls, lay.total);
 
@@ -464,7 +468,7 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
     s->maxch = maxch;
     s->wavesPerBlock = (int)std::min<size_t>(4, std::max<size_t>(1, kMaxLdsPerBlock / lay.total));
     int bpc = 0, cus = 0;
-    hipError_t he = beamFlatOccupancy(maxch, fast ? fastMode(g->n, g->G) : 0, s->wavesPerBlock, (size_t)lay.total * s->wavesPerBlock, &bpc);
+    hipError_t he = beamFlatOccupancy(maxch, fastKind, s->wavesPerBlock, (size_t)lay.total * s->wavesPerBlock, &bpc);
     r = deviceCUs(&cus);
     if (he != hipSuccess || r != PANN_OK) {
         delete s;
@@ -488,7 +492,7 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
     L.capV = capV;
     L.ldsPerWave = lay.total;
     L.lds = lay;
-    L.fast = fast ? fastMode(g->n, g->G) : 0;
+    L.fast = fastKind;
     L.totalWaves = totalWaves;
     L.queue = nullptr;
     if (!ldsFilter) {

@@ -19,7 +19,7 @@ struct pann_batch {
     pann_batch_config cfg{};
     uint32_t maxQueries = 0, nq = 0, dim = 0;
     int maxch = 1, wr = 1, wavesPerBlock = 1, blocks = 0;
-    bool fast = false;  // flat: register-resident kernel
+    int fastKind = 0;  // flat: FlatLaunch::fast of the register-resident kernel (0: general)
     uint32_t totalWaves = 0;
     FlatLaunch L{};
     HnswLaunch H{};
@@ -86,9 +86,7 @@ int finishSizing(pann_batch* b, uint32_t ldsPerWave, int gbits) {
     b->cfg.waves_per_block = wpb;
     int bpc = 0;
     hipError_t e = b->kind == 0
-                       ? beamFlatOccupancy(b->maxch,
-                                           b->fast ? fastMode(b->graph->n, b->graph->G) + (b->cfg.prefetch ? 2 : 0) : 0,
-                                           wpb, (size_t)ldsPerWave * wpb, &bpc)
+                       ? beamFlatOccupancy(b->maxch, b->fastKind, wpb, (size_t)ldsPerWave * wpb, &bpc)
                        : beamHnswOccupancy(b->wr, b->maxch, wpb, (size_t)ldsPerWave * wpb, &bpc);
     if (e != hipSuccess) return setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
     b->blocks = std::max(1, bpc) * cus;
@@ -123,15 +121,18 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
     const uint32_t capV = cfg->visited_slots ? cfg->visited_slots : flatVisitedCap(LsMax);
     b->cfg.lds_filter_bits = fbLds;
     b->cfg.visited_slots = capV;
+    // one frontier key per lane (filter in LDS), or two (64 < max_Ls <= 128, filter in HBM)
     const bool fast = b->maxch == 1 && LsMax <= kFastLsMax && fbLds >= kFastFilterBits;
-    b->fast = fast;
-    if (!fast) b->cfg.prefetch = 0;  // only the register-resident form prefetches
-    const FlatLds lay = fast ? flatFastLdsLayout(capV) : flatLdsLayout(LsMax, b->maxch, capV, fbLds);
+    const bool fast2 = b->maxch == 1 && !fast && LsMax <= kFast2LsMax;
+    if (!fast) b->cfg.prefetch = 0;  // only the one-key form prefetches
+    b->fastKind = fast ? fastMode(g->n, g->G) + (b->cfg.prefetch ? 2 : 0) : fast2 ? 8 + fastMode(g->n, g->G) : 0;
+    const FlatLds lay = fast ? flatFastLdsLayout(capV)
+                      : fast2 ? flatFast2LdsLayout(capV) : flatLdsLayout(LsMax, b->maxch, capV, fbLds);
     if (lay.total > kMaxLdsPerBlock) {
         delete b;
         return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsMax, lay.total);
     }
-    const int gbits = fbMax > fbLds ? fbMax : 0;
+    const int gbits = (fbMax > fbLds || fast2) ? fbMax : 0;
     r = finishSizing(b, lay.total, gbits);
     if (r != PANN_OK) {
         pann_batch_free(b);
@@ -146,7 +147,7 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
     L.capV = capV;
     L.ldsPerWave = lay.total;
     L.lds = lay;
-    L.fast = fast ? fastMode(g->n, g->G) + (cfg->prefetch ? 2 : 0) : 0;
+    L.fast = b->fastKind;
     L.gfilt = b->gfilt;
     L.queue = b->queue;
     L.order = b->order;

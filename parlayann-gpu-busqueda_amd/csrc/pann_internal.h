@@ -97,6 +97,21 @@ __host__ __device__ inline FlatLds flatFastLdsLayout(uint32_t capV) {
     return L;
 }
 
+// The two-register form (64 < Ls <= 128) keeps its seen filter in HBM; LDS holds the
+// expansion counts and the byte tags, which double as a 128-key compaction buffer.
+constexpr int kFast2LsMax = 128;
+__host__ __device__ inline FlatLds flatFast2LdsLayout(uint32_t capV) {
+    static_assert(8 * kFast2LsMax <= kTagSlots, "compaction buffer must fit in the tag bytes");
+    FlatLds L = {};
+    uint32_t o = 0;
+    L.vkey = o; o += 4u * capV;
+    L.tag = o;  o += kTagSlots;
+    L.kkey = L.tag;
+    L.vcnt = o; o += alignUp(capV, 8);
+    L.total = alignUp(o, 16);
+    return L;
+}
+
 // bits of ParlayANN's seen filter: max(10, ceil(log2(Ls*Ls)) - 2)   cuda/busqueda.cu:536-538
 __host__ __device__ inline int ceilLog2(uint64_t x) {
     int b = 0;
@@ -132,7 +147,9 @@ struct FlatLaunch {
     FlatLds lds;
     int32_t fast;              // register-resident kernel (layout flatFastLdsLayout): 1, or
                                // 2 when base and adjacency are below 4 GiB; +2 with next-row
-                               // prefetch (the launcher adds 4 for normsInRow); 0: general
+                               // prefetch (the launcher adds 4 for normsInRow).  Two-register
+                               // form (flatFast2LdsLayout, filter in gfilt): 9, or 10 below
+                               // 4 GiB (the launcher adds 2 for normsInRow).  0: general
     uint32_t* gfilt;           // per-wave global filter regions
     uint32_t* queue;           // persistent mode: {next ticket, finished waves}; null: static
     const uint32_t* order;     // persistent mode: ticket -> query (null: identity)

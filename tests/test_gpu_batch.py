@@ -148,11 +148,13 @@ def test_autotune_returns_clean_config(gpu, small):
     batch.free(), g.free(), b.free()
 
 
-@pytest.mark.parametrize("prefetch,wide", [(0, False), (1, False), (1, True)])
-def test_register_resident_batch_forms(gpu, w1, monkeypatch, prefetch, wide):
-    """Batches with max_Ls <= 64 run the register-resident kernel: with and without the
-    next-row prefetch, with 32-bit and 64-bit addressing, on uneven Ls / k / visit limits
-    and on a graph with duplicate edges."""
+@pytest.mark.parametrize("prefetch,wide,max_Ls", [(0, False, 64), (1, False, 64), (1, True, 64),
+                                                  (0, False, 128), (0, True, 128)])
+def test_register_resident_batch_forms(gpu, w1, monkeypatch, prefetch, wide, max_Ls):
+    """Batches with max_Ls <= 64 (one frontier key per lane) or <= 128 (two keys per lane,
+    seen filter in HBM) run the register-resident kernels: with and without the next-row
+    prefetch, with 32-bit and 64-bit addressing, on uneven Ls / k / visit limits and on a
+    graph with duplicate edges."""
     from oracle import oracle as O
     from pann import datagen, search
     from conftest import random_graph
@@ -164,14 +166,14 @@ def test_register_resident_batch_forms(gpu, w1, monkeypatch, prefetch, wide):
     b2, q2 = datagen.gmm_dataset(3000, 150, dim=100, ncomp=6, seeds=(21, 22, 23))
     cases.append((b2, q2, random_graph(3000, 64, np.random.default_rng(64), dup_frac=0.2, min_degree=21)))
     for base, queries, graph in cases:
-        params = _mixed_params(len(queries), len(base), rng, ls_lo=1, ls_hi=64, ks=(1, 10, 50))
+        params = _mixed_params(len(queries), len(base), rng, ls_lo=1, ls_hi=max_Ls, ks=(1, 10, 50))
         b, g = search.Vectors(base), search.FlatGraph(graph)
-        batch = search.Batch(len(queries), 64, 50, base=b, graph=g, prefetch=prefetch)
-        assert batch.config()[0]["prefetch"] == prefetch
+        batch = search.Batch(len(queries), max_Ls, 50, base=b, graph=g, prefetch=prefetch)
+        assert batch.config()[0]["prefetch"] == (prefetch if max_Ls <= 64 else 0)
         batch.set_queries(queries, params)
         batch.launch()
         search.synchronize()
         r = batch.download()
         o = O.beam_flat(base, graph, queries, _oracle_params(params))
-        _check_against_oracle(r, o, params, f"prefetch={prefetch} wide={wide}")
+        _check_against_oracle(r, o, params, f"prefetch={prefetch} wide={wide} max_Ls={max_Ls}")
         batch.free(), g.free(), b.free()

@@ -130,7 +130,7 @@ def test_wide_addressing_form(gpu, w1, monkeypatch):
     vectors or the adjacency exceed 4 GiB), forced on small indexes."""
     monkeypatch.setenv("PANN_WIDE_ADDRESSING", "1")
     base, queries, graph = w1
-    for Ls in (10, 40):
+    for Ls in (10, 40, 100):
         r = _run(base, graph, queries, Ls, 10)
         o = _oracle(base, graph, queries, Ls, 10)
         _assert_same(r, o, 10, f"wide W1 Ls={Ls}")
@@ -141,3 +141,20 @@ def test_wide_addressing_form(gpu, w1, monkeypatch):
     r = _run(base, graph, queries, 33, 10)
     o = _oracle(base, graph, queries, 33, 10)
     _assert_same(r, o, 10, "wide dup G=64 Ls=33")
+
+
+@pytest.mark.parametrize("dim", [124, 128])
+def test_extreme_distances(gpu, dim):
+    """Vectors of +-127/-128 only: distances around and above 2^22 (the widest the packed
+    frontier keys carry is 2^23 - 1 > 128 * 255^2), with and without the norm in the row."""
+    rng = np.random.default_rng(dim)
+    n = 3000
+    # base mostly -128, queries mostly 127: even nearest neighbours lie beyond 2^22
+    base = np.where(rng.random((n, dim)) < 0.9, -128, 127).astype(np.int8)
+    queries = np.where(rng.random((120, dim)) < 0.9, 127, -128).astype(np.int8)
+    graph = random_graph(n, 48, rng, dup_frac=0.05, min_degree=16)
+    for Ls in (10, 40, 100):
+        r = _run(base, graph, queries, Ls, 10)
+        o = _oracle(base, graph, queries, Ls, 10)
+        assert int(o.dists[:, :10].max()) > (1 << 22)
+        _assert_same(r, o, 10, f"extreme dim={dim} Ls={Ls}")
