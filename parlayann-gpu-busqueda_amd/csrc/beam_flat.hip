@@ -1105,8 +1105,10 @@ __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
 
     Prof P;
     P.start();
+    // queue mode ends on kQueueEmpty, strided mode past the last query
+    const uint32_t qEnd = L.queue ? kQueueEmpty : L.nq;
     uint32_t q = L.queue ? takeQuery(L.queue, L.order, L.nq, lane) : waveGlobal;
-    while (q < L.nq) {
+    while (q < qEnd) {
         QueryCfg c;
         if (L.qp) {
             const QueryParams& P = L.qp[q];

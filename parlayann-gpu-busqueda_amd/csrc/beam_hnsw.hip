@@ -347,8 +347,10 @@ __global__ __launch_bounds__(256) void beamHnswKernel(HnswLaunch L) {
     unsigned char* lds = smem + (size_t)waveInBlock * L.ldsPerWave;
     uint32_t* gfilt = L.gfilt ? L.gfilt + ((size_t)waveGlobal << L.gbitsMax) : nullptr;
 
+    // queue mode ends on kQueueEmpty, strided mode past the last query
+    const uint32_t qEnd = L.queue ? kQueueEmpty : L.nq;
     uint32_t q = L.queue ? takeQuery(L.queue, L.order, L.nq, lane) : waveGlobal;
-    while (q < L.nq) {
+    while (q < qEnd) {
         int Ls = L.Ls, k = L.k;
         int64_t visit = L.visit;
         if (L.qp) {

@@ -408,14 +408,18 @@ __device__ __forceinline__ bool seenFilterChunk(uint32_t* filt, unsigned char* t
     return seenFilterEnd<G>(filt, s, a, part, lane, lanesBelow);
 }
 
+// What takeQuery returns once the queue is drained.  The order array may hold a range of
+// a larger batch, so a query index says nothing about the queue's end.
+constexpr uint32_t kQueueEmpty = 0xFFFFFFFFu;
+
 // Persistent work queue: queue[0] is the next ticket, queue[1] counts finished waves.
-// Tickets map to queries through `order` (NULL: identity); nq means "drained".
+// Tickets below nq map to queries through `order` (NULL: identity).
 __device__ __forceinline__ uint32_t takeQuery(uint32_t* queue, const uint32_t* order, uint32_t nq,
                                               int lane) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(&queue[0], 1u);
     t = (uint32_t)readlane((int)t, 0);
-    if (t >= nq) return nq;
+    if (t >= nq) return kQueueEmpty;
     return order ? order[t] : t;
 }
 

@@ -1,7 +1,9 @@
-// pann_batch.cpp -- the batched persistent form: one launch of resident wavefronts drains
-// a device queue of queries that each carry their own qid, Ls, k, visit limit and cut;
-// the launch can be captured into a HIP graph and replayed; an autotuner picks the
-// launch shape per index.
+// pann_batch.cpp -- the batched persistent form: resident wavefronts drain a device queue
+// of queries that each carry their own qid, Ls, k, visit limit and cut; the launch can be
+// captured into a HIP graph and replayed; an autotuner picks the launch shape per index.
+// A flat batch whose max_Ls exceeds 64 is split by Ls class (<= 64, <= 128, the rest):
+// each class drains in a launch of its own, sized for the longest beam of that class, so
+// short beams run at the occupancy of their own LDS footprint rather than max_Ls's.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -11,6 +13,16 @@
 
 using namespace pann;
 
+// One launch of a flat batch: the queries with LsLo < Ls <= LsHi, a contiguous range of the
+// Ls-descending order, drained by a kernel sized for LsHi.
+struct FlatPart {
+    int LsLo = 0, LsHi = 0;
+    int fastKind = 0;  // FlatLaunch::fast of the register-resident kernel (0: general)
+    int wavesPerBlock = 1, blocks = 0;
+    uint32_t begin = 0, count = 0;
+    FlatLaunch L{};
+};
+
 struct pann_batch {
     int kind = 0;  // 0 flat, 1 hnsw
     const pann_vectors* base = nullptr;
@@ -19,9 +31,8 @@ struct pann_batch {
     pann_batch_config cfg{};
     uint32_t maxQueries = 0, nq = 0, dim = 0;
     int maxch = 1, wr = 1, wavesPerBlock = 1, blocks = 0;
-    int fastKind = 0;  // flat: FlatLaunch::fast of the register-resident kernel (0: general)
     uint32_t totalWaves = 0;
-    FlatLaunch L{};
+    std::vector<FlatPart> parts;  // flat: one per Ls class, ascending LsHi
     HnswLaunch H{};
     int8_t* queries = nullptr;
     QueryParams* params = nullptr;
@@ -33,6 +44,7 @@ struct pann_batch {
     hipGraph_t graphDef = nullptr;
     hipGraphExec_t graphExec = nullptr;
     uint32_t capturedNq = 0;
+    std::vector<uint32_t> capturedCounts;
 };
 
 namespace {
@@ -43,6 +55,7 @@ void releaseGraph(pann_batch* b) {
     b->graphExec = nullptr;
     b->graphDef = nullptr;
     b->capturedNq = 0;
+    b->capturedCounts.clear();
 }
 
 int checkBatchConfig(const pann_batch_config* cfg) {
@@ -67,14 +80,27 @@ int allocCommon(pann_batch* b) {
     HIP_TRY(hipMemset(b->queries, 0, mq * kVecBytes));
     HIP_TRY(hipMalloc(&b->params, mq * sizeof(QueryParams)));
     HIP_TRY(hipMalloc(&b->order, mq * 4));
-    HIP_TRY(hipMalloc(&b->queue, 2 * 4));
-    HIP_TRY(hipMemset(b->queue, 0, 2 * 4));
+    const size_t queues = std::max<size_t>(1, b->parts.size());  // one counter pair per launch
+    HIP_TRY(hipMalloc(&b->queue, queues * 2 * 4));
+    HIP_TRY(hipMemset(b->queue, 0, queues * 2 * 4));
     int r = allocOutputs(b->out, b->maxQueries, (uint32_t)b->cfg.max_k);
     if (r != PANN_OK) return r;
     return PANN_OK;
 }
 
-int finishSizing(pann_batch* b, uint32_t ldsPerWave, int gbits) {
+std::vector<uint32_t> partCounts(const pann_batch* b) {
+    std::vector<uint32_t> c;
+    for (const FlatPart& P : b->parts) c.push_back(P.count);
+    return c;
+}
+
+int allocFilters(pann_batch* b, size_t bytes) {
+    if (bytes) HIP_TRY(hipMalloc(&b->gfilt, bytes));
+    return PANN_OK;
+}
+
+// Wavefronts per block and resident blocks for one kernel of the batch.
+int sizeLaunch(pann_batch* b, int fastKind, uint32_t ldsPerWave, int* wpbOut, int* blocksOut) {
     int cus = 0;
     int r = deviceCUs(&cus);
     if (r != PANN_OK) return r;
@@ -82,17 +108,55 @@ int finishSizing(pann_batch* b, uint32_t ldsPerWave, int gbits) {
     if (wpb == 0) wpb = (int)std::min<size_t>(4, std::max<size_t>(1, kMaxLdsPerBlock / ldsPerWave));
     if ((size_t)wpb * ldsPerWave > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "%d wavefronts x %u bytes of LDS exceed 64 KB", wpb, ldsPerWave);
-    b->wavesPerBlock = wpb;
-    b->cfg.waves_per_block = wpb;
     int bpc = 0;
     hipError_t e = b->kind == 0
-                       ? beamFlatOccupancy(b->maxch, b->fastKind, wpb, (size_t)ldsPerWave * wpb, &bpc)
+                       ? beamFlatOccupancy(b->maxch, fastKind, wpb, (size_t)ldsPerWave * wpb, &bpc)
                        : beamHnswOccupancy(b->wr, b->maxch, wpb, (size_t)ldsPerWave * wpb, &bpc);
     if (e != hipSuccess) return setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
-    b->blocks = std::max(1, bpc) * cus;
-    b->totalWaves = (uint32_t)b->blocks * wpb;
-    if (gbits > 0) HIP_TRY(hipMalloc(&b->gfilt, ((size_t)b->totalWaves << gbits) * 4));
-    return allocCommon(b);
+    *wpbOut = wpb;
+    *blocksOut = std::max(1, bpc) * cus;
+    return PANN_OK;
+}
+
+// Sizes one flat launch for beams up to LsHi under the batch's configuration.  The visited
+// hash and the LDS filter never exceed what LsHi itself derives.  Records the launch's
+// global filter bits (0: none) in *gbits.
+int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, FlatPart* P, int* gbits) {
+    const pann_batch_config& c = b->cfg;
+    const int fbMax = flatFilterBits(LsHi);
+    const int fbLds = std::min(c.lds_filter_bits ? c.lds_filter_bits : kFilterBitsLds, fbMax);
+    const uint32_t derived = flatVisitedCap(LsHi);
+    const uint32_t capV = !c.visited_slots ? derived : top ? c.visited_slots : std::min(c.visited_slots, derived);
+    // one frontier key per lane (filter in LDS), or two (64 < Ls <= 128, filter in HBM)
+    const bool fast = b->maxch == 1 && LsHi <= kFastLsMax && fbLds >= kFastFilterBits;
+    const bool fast2 = b->maxch == 1 && !fast && LsHi <= kFast2LsMax;
+    const int mode = fastMode(b->graph->n, b->graph->G);
+    P->LsLo = LsLo;
+    P->LsHi = LsHi;
+    P->fastKind = fast ? mode + (c.prefetch ? 2 : 0) : fast2 ? 8 + mode : 0;
+    const FlatLds lay = fast ? flatFastLdsLayout(capV)
+                      : fast2 ? flatFast2LdsLayout(capV) : flatLdsLayout(LsHi, b->maxch, capV, fbLds);
+    if (lay.total > kMaxLdsPerBlock)
+        return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsHi, lay.total);
+    int r = sizeLaunch(b, P->fastKind, lay.total, &P->wavesPerBlock, &P->blocks);
+    if (r != PANN_OK) return r;
+    *gbits = (fbMax > fbLds || fast2) ? fbMax : 0;
+    FlatLaunch& L = P->L;
+    L.n = b->graph->n;
+    L.G = b->graph->G;
+    L.LsMax = LsHi;
+    L.fbitsLds = fbLds;
+    L.gbitsMax = *gbits;
+    L.capV = capV;
+    L.ldsPerWave = lay.total;
+    L.lds = lay;
+    L.fast = P->fastKind;
+    if (top) {
+        b->cfg.lds_filter_bits = fbLds;
+        b->cfg.visited_slots = capV;
+        if (!fast) b->cfg.prefetch = 0;
+    }
+    return PANN_OK;
 }
 
 }  // namespace
@@ -116,55 +180,53 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
     b->dim = base->dim;
     b->maxch = g->G <= 64 ? 1 : g->G <= 128 ? 2 : 4;
     const int LsMax = cfg->max_Ls;
-    const int fbMax = flatFilterBits(LsMax);
-    const int fbLds = std::min(cfg->lds_filter_bits ? cfg->lds_filter_bits : kFilterBitsLds, fbMax);
-    const uint32_t capV = cfg->visited_slots ? cfg->visited_slots : flatVisitedCap(LsMax);
-    b->cfg.lds_filter_bits = fbLds;
-    b->cfg.visited_slots = capV;
-    // one frontier key per lane (filter in LDS), or two (64 < max_Ls <= 128, filter in HBM)
-    const bool fast = b->maxch == 1 && LsMax <= kFastLsMax && fbLds >= kFastFilterBits;
-    const bool fast2 = b->maxch == 1 && !fast && LsMax <= kFast2LsMax;
-    if (!fast) b->cfg.prefetch = 0;  // only the one-key form prefetches
-    b->fastKind = fast ? fastMode(g->n, g->G) + (b->cfg.prefetch ? 2 : 0) : fast2 ? 8 + fastMode(g->n, g->G) : 0;
-    const FlatLds lay = fast ? flatFastLdsLayout(capV)
-                      : fast2 ? flatFast2LdsLayout(capV) : flatLdsLayout(LsMax, b->maxch, capV, fbLds);
-    if (lay.total > kMaxLdsPerBlock) {
-        delete b;
-        return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsMax, lay.total);
+    if (LsMax > kFastLsMax) b->cfg.prefetch = 0;  // only the one-key form prefetches
+    std::vector<int> his;
+    for (int hi : {kFastLsMax, kFast2LsMax})
+        if (hi < LsMax) his.push_back(hi);
+    his.push_back(LsMax);
+    b->parts.resize(his.size());
+    size_t gfiltBytes = 0;
+    for (size_t i = 0; i < his.size(); i++) {
+        int gbits = 0;
+        r = planFlatPart(b, i ? his[i - 1] : 0, his[i], i + 1 == his.size(), &b->parts[i], &gbits);
+        if (r != PANN_OK) {
+            pann_batch_free(b);
+            return r;
+        }
+        const FlatPart& P = b->parts[i];
+        const uint32_t waves = (uint32_t)P.blocks * P.wavesPerBlock;
+        b->totalWaves = std::max(b->totalWaves, waves);
+        if (gbits) gfiltBytes = std::max(gfiltBytes, ((size_t)waves << gbits) * 4);
     }
-    const int gbits = (fbMax > fbLds || fast2) ? fbMax : 0;
-    r = finishSizing(b, lay.total, gbits);
+    const FlatPart& top = b->parts.back();
+    b->wavesPerBlock = top.wavesPerBlock;
+    b->blocks = top.blocks;
+    b->cfg.waves_per_block = top.wavesPerBlock;
+    // the launches run one after another on one stream and share the global filters
+    r = allocFilters(b, gfiltBytes);
+    if (r == PANN_OK) r = allocCommon(b);
     if (r != PANN_OK) {
         pann_batch_free(b);
         return r;
     }
-    FlatLaunch& L = b->L;
-    L.n = g->n;
-    L.G = g->G;
-    L.LsMax = LsMax;
-    L.fbitsLds = fbLds;
-    L.gbitsMax = gbits;
-    L.capV = capV;
-    L.ldsPerWave = lay.total;
-    L.lds = lay;
-    L.fast = b->fastKind;
-    L.gfilt = b->gfilt;
-    L.queue = b->queue;
-    L.order = b->order;
-    L.totalWaves = b->totalWaves;
-    L.qp = b->params;
-    L.qids = nullptr;
-    L.queries = b->queries;
-    L.base = base->dev;
-    L.normsInRow = base->dim <= (uint32_t)kNormInRowMaxDim;
-    L.adj = g->adj;
-    L.outIds = b->out.ids;
-    L.outDists = b->out.dists;
-    L.outStride = b->out.stride;
-    L.outComps = b->out.comps;
-    L.outFlags = b->out.flags;
-    L.outEdges = b->out.edges;
-    L.outHops = b->out.hops;
+    for (FlatPart& P : b->parts) {
+        FlatLaunch& L = P.L;
+        L.gfilt = b->gfilt;
+        L.qp = b->params;
+        L.qids = nullptr;
+        L.queries = b->queries;
+        L.base = base->dev;
+        L.normsInRow = base->dim <= (uint32_t)kNormInRowMaxDim;
+        L.adj = g->adj;
+        L.outIds = b->out.ids;
+        L.outDists = b->out.dists;
+        L.outStride = b->out.stride;
+        L.outComps = b->out.comps;
+        L.outFlags = b->out.flags;
+        L.outEdges = b->out.edges;
+        L.outHops = b->out.hops;
+    }
     *out = b;
     return PANN_OK;
 }
@@ -206,7 +268,13 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
         return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsMax, lay.total);
     }
     const int gbits = fbMax > fbLds ? fbMax : 0;
-    r = finishSizing(b, lay.total, gbits);
+    r = sizeLaunch(b, 0, lay.total, &b->wavesPerBlock, &b->blocks);
+    if (r == PANN_OK) {
+        b->cfg.waves_per_block = b->wavesPerBlock;
+        b->totalWaves = (uint32_t)b->blocks * b->wavesPerBlock;
+        r = allocFilters(b, gbits ? ((size_t)b->totalWaves << gbits) * 4 : 0);
+    }
+    if (r == PANN_OK) r = allocCommon(b);
     if (r != PANN_OK) {
         pann_batch_free(b);
         return r;
@@ -266,6 +334,15 @@ int pann_batch_set_queries(pann_batch* b, const int8_t* host_rows, const pann_qu
     std::iota(b->hostOrder.begin(), b->hostOrder.end(), 0u);
     std::stable_sort(b->hostOrder.begin(), b->hostOrder.end(),
                      [&](uint32_t x, uint32_t y) { return params[x].Ls > params[y].Ls; });
+    // each flat launch takes the range of that order whose Ls falls in its class
+    for (FlatPart& P : b->parts) {
+        P.begin = P.count = 0;
+        for (uint32_t i = 0; i < nq; i++) {
+            const int Ls = params[i].Ls;
+            P.begin += Ls > P.LsHi;
+            P.count += Ls > P.LsLo && Ls <= P.LsHi;
+        }
+    }
     HIP_TRY(hipMemcpy2DAsync(b->queries, kVecBytes, host_rows, b->dim, b->dim, nq, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->params, params, (size_t)nq * sizeof(QueryParams), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->order, b->hostOrder.data(), (size_t)nq * 4, hipMemcpyHostToDevice, st));
@@ -276,20 +353,27 @@ int pann_batch_set_queries(pann_batch* b, const int8_t* host_rows, const pann_qu
 int pann_batch_launch(pann_batch* b, void* stream) {
     if (!b) return setError(PANN_ERR_ARG, "batch is NULL");
     if (b->nq == 0) return PANN_OK;
-    // a small batch launches only as many wavefronts as it has queries
-    const int blocks = (int)std::min<uint32_t>((uint32_t)b->blocks,
-                                               (b->nq + b->wavesPerBlock - 1) / b->wavesPerBlock);
-    const uint32_t waves = (uint32_t)blocks * b->wavesPerBlock;
-    hipError_t e;
+    hipError_t e = hipSuccess;
     if (b->kind == 0) {
-        FlatLaunch L = b->L;
-        L.nq = b->nq;
-        L.totalWaves = waves;
-        e = launchBeamFlat(L, b->maxch, b->wavesPerBlock, blocks, (hipStream_t)stream);
+        // longest class first; a small class launches only as many wavefronts as it has queries
+        for (size_t i = b->parts.size(); i-- > 0 && e == hipSuccess;) {
+            const FlatPart& P = b->parts[i];
+            if (P.count == 0) continue;
+            const int blocks = (int)std::min<uint32_t>((uint32_t)P.blocks,
+                                                       (P.count + P.wavesPerBlock - 1) / P.wavesPerBlock);
+            FlatLaunch L = P.L;
+            L.nq = P.count;
+            L.order = b->order + P.begin;
+            L.queue = b->queue + 2 * i;
+            L.totalWaves = (uint32_t)blocks * P.wavesPerBlock;
+            e = launchBeamFlat(L, b->maxch, P.wavesPerBlock, blocks, (hipStream_t)stream);
+        }
     } else {
+        const int blocks = (int)std::min<uint32_t>((uint32_t)b->blocks,
+                                                   (b->nq + b->wavesPerBlock - 1) / b->wavesPerBlock);
         HnswLaunch H = b->H;
         H.nq = b->nq;
-        H.totalWaves = waves;
+        H.totalWaves = (uint32_t)blocks * b->wavesPerBlock;
         e = launchBeamHnsw(H, b->wr, b->maxch, b->wavesPerBlock, blocks, (hipStream_t)stream);
     }
     if (e != hipSuccess) return setError(PANN_ERR_HIP, "batched launch: %s", hipGetErrorString(e));
@@ -313,13 +397,15 @@ int pann_batch_capture(pann_batch* b, void* stream) {
     b->graphDef = g;
     HIP_TRY(hipGraphInstantiate(&b->graphExec, g, nullptr, nullptr, 0));
     b->capturedNq = b->nq;
+    b->capturedCounts = partCounts(b);
     return PANN_OK;
 }
 
 int pann_batch_replay(pann_batch* b, void* stream) {
     if (!b) return setError(PANN_ERR_ARG, "batch is NULL");
     if (!b->graphExec) return setError(PANN_ERR_ARG, "no captured launch");
-    if (b->capturedNq != b->nq) return setError(PANN_ERR_ARG, "queries changed since the capture");
+    if (b->capturedNq != b->nq || b->capturedCounts != partCounts(b))
+        return setError(PANN_ERR_ARG, "queries changed since the capture");
     HIP_TRY(hipGraphLaunch(b->graphExec, (hipStream_t)stream));
     return PANN_OK;
 }

@@ -177,3 +177,55 @@ def test_register_resident_batch_forms(gpu, w1, monkeypatch, prefetch, wide, max
         o = O.beam_flat(base, graph, queries, _oracle_params(params))
         _check_against_oracle(r, o, params, f"prefetch={prefetch} wide={wide} max_Ls={max_Ls}")
         batch.free(), g.free(), b.free()
+
+
+def test_ls_class_launches(gpu, w1):
+    """A flat batch with max_Ls above 64 drains each Ls class (<= 64, 65..128, above 128)
+    in a launch of its own: every class alone, all three mixed, rows wider than 64 entries,
+    and a captured graph that refuses to replay once the class sizes change."""
+    from oracle import oracle as O
+    from pann import datagen, search
+    from conftest import random_graph
+    base, queries, graph = w1
+    nq = 400
+    rng = np.random.default_rng(31)
+    b, g = search.Vectors(base), search.FlatGraph(graph)
+    batch = search.Batch(nq, 300, 20, base=b, graph=g)
+    for lo, hi in [(1, 64), (65, 128), (129, 300), (1, 300)]:
+        params = _mixed_params(nq, len(base), rng, ls_lo=lo, ls_hi=hi, ks=(1, 10, 20))
+        batch.set_queries(queries[:nq], params)
+        batch.launch()
+        search.synchronize()
+        r = batch.download()
+        o = O.beam_flat(base, graph, queries[:nq], _oracle_params(params))
+        _check_against_oracle(r, o, params, f"Ls {lo}..{hi}")
+    # the captured graph holds the class ranges of the queries it was captured with
+    st = search.Stream()
+    batch.capture(st.ptr)
+    batch.replay(st.ptr)
+    st.synchronize()
+    assert np.array_equal(batch.download().ids, r.ids)
+    params = _mixed_params(nq, len(base), rng, ls_lo=1, ls_hi=64, ks=(1, 10, 20))
+    batch.set_queries(queries[:nq], params, st.ptr)
+    with pytest.raises(RuntimeError, match="changed since the capture"):
+        batch.replay(st.ptr)
+    batch.capture(st.ptr)
+    batch.replay(st.ptr)
+    st.synchronize()
+    r = batch.download()
+    o = O.beam_flat(base, graph, queries[:nq], _oracle_params(params))
+    _check_against_oracle(r, o, params, "recaptured")
+    batch.free(), st.free(), g.free(), b.free()
+    # rows of up to 100 entries: every class runs the general kernel
+    b2, q2 = datagen.gmm_dataset(3000, 200, dim=100, ncomp=6, seeds=(24, 25, 26))
+    g2 = random_graph(3000, 100, np.random.default_rng(100), dup_frac=0.1, min_degree=30)
+    params = _mixed_params(len(q2), len(b2), rng, ls_lo=5, ls_hi=250, ks=(1, 10, 20))
+    b, g = search.Vectors(b2), search.FlatGraph(g2)
+    batch = search.Batch(len(q2), 250, 20, base=b, graph=g)
+    batch.set_queries(q2, params)
+    batch.launch()
+    search.synchronize()
+    r = batch.download()
+    o = O.beam_flat(b2, g2, q2, _oracle_params(params))
+    _check_against_oracle(r, o, params, "G=100")
+    batch.free(), g.free(), b.free()
