@@ -24,6 +24,53 @@ __device__ __forceinline__ uint32_t visitedSlot(uint32_t id, uint32_t logCap) {
     return (id * 2654435761u) >> (32 - logCap);
 }
 
+// Visited multiset of the general form: capV id slots (kEmpty: free) and their expansion
+// counts (u8), in LDS or in a per-wave HBM region.  An id hashes to an aligned group of
+// four slots and takes the first free slot from there on, group by group, so one 16-B key
+// load and one 4-B count load answer most lookups.  Inserts stop one short of capV, so
+// every walk meets a free slot.
+__device__ __forceinline__ uint32_t visitedCount(const uint32_t* vkey, const unsigned char* vcnt,
+                                                 uint32_t id, uint32_t logCap, uint32_t vmask) {
+    uint32_t g = visitedSlot(id, logCap) & ~3u;
+    while (true) {
+        const uint4 k4 = *reinterpret_cast<const uint4*>(vkey + g);
+        const uint32_t c4 = *reinterpret_cast<const uint32_t*>(vcnt + g);
+        const uint32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (kk[j] == id) return (c4 >> (8 * j)) & 0xFFu;
+            if (kk[j] == kEmpty) return 0;
+        }
+        g = (g + 4) & vmask;
+    }
+}
+
+// One more expansion of id (one lane).  Returns false when a new id would fill the table.
+__device__ __forceinline__ bool visitedInsert(uint32_t* vkey, unsigned char* vcnt, uint32_t id,
+                                              uint32_t logCap, uint32_t vmask, uint32_t capV,
+                                              uint32_t& distinct) {
+    uint32_t g = visitedSlot(id, logCap) & ~3u;
+    while (true) {
+        const uint4 k4 = *reinterpret_cast<const uint4*>(vkey + g);
+        const uint32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (kk[j] == id) {
+                vcnt[g + j] = (unsigned char)(vcnt[g + j] + 1);
+                return true;
+            }
+            if (kk[j] == kEmpty) {
+                if (distinct + 1 >= capV) return false;
+                vkey[g + j] = id;
+                vcnt[g + j] = 1;
+                distinct++;
+                return true;
+            }
+        }
+        g = (g + 4) & vmask;
+    }
+}
+
 struct QueryCfg {
     uint32_t qid;
     int Ls, k;
@@ -46,9 +93,23 @@ __device__ __forceinline__ void makeUniform(QueryCfg& c) {
         (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)cb)));
 }
 
-template <int MAXCH, bool FGLOBAL>
+// Candidate keys carry the expansion count of their id in the low bits,
+//     d (23 bits) | id (32 bits) | count (9 bits, saturated),
+// so the count is looked up next to the candidate's vector load and travels with it
+// through the sort (equal (d, id) means equal count, so the order by (d, id) stands).
+// Frontier keys are (d << 32 | id).
+__device__ __forceinline__ uint64_t countKey(int d, uint32_t id, uint32_t cnt) {
+    return ((uint64_t)(uint32_t)d << 41) | ((uint64_t)id << 9) | (cnt < 511u ? cnt : 511u);
+}
+__device__ __forceinline__ uint64_t frontierOfCountKey(uint64_t k) {
+    return ((k >> 41) << 32) | ((k >> 9) & 0xFFFFFFFFull);
+}
+
+// VGLOBAL: the visited table lives in this wave's HBM region (gvk / gvc) rather than in
+// LDS, for beams whose table would otherwise bound how many wavefronts share a CU.
+template <int MAXCH, bool FGLOBAL, bool VGLOBAL>
 __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, const QueryCfg& c,
-                          uint32_t* gfilt, Prof& P) {
+                          uint32_t* gfilt, uint32_t* gvk, unsigned char* gvc, Prof& P) {
     const int lane = laneId();
     const int grp = lane >> 3, sub = lane & 7;
     const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -61,8 +122,8 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
     unsigned char* kvis = lds + L.lds.kvis;
     uint32_t* pend = reinterpret_cast<uint32_t*>(lds + L.lds.pend);
     unsigned char* tag = lds + L.lds.tag;
-    uint32_t* vkey = reinterpret_cast<uint32_t*>(lds + L.lds.vkey);
-    unsigned char* vcnt = lds + L.lds.vcnt;
+    uint32_t* vkey = VGLOBAL ? gvk : reinterpret_cast<uint32_t*>(lds + L.lds.vkey);
+    unsigned char* vcnt = VGLOBAL ? gvc : lds + L.lds.vcnt;
     uint32_t* filt = FGLOBAL ? gfilt : reinterpret_cast<uint32_t*>(lds + L.lds.filt);
 
     const int Ls = c.Ls;
@@ -127,17 +188,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
         bool overflow = false;
         if (lane == 0) {
             fvis[cur][u] = 1;
-            uint32_t h = visitedSlot(uid, logCap);
-            while (vkey[h] != kEmpty && vkey[h] != uid) h = (h + 1) & vmask;
-            if (vkey[h] == uid) {
-                vcnt[h] = (unsigned char)(vcnt[h] + 1);
-            } else if (distinctVisited + 1 < capV) {
-                vkey[h] = uid;
-                vcnt[h] = 1;
-                distinctVisited++;
-            } else {
-                overflow = true;
-            }
+            overflow = !visitedInsert(vkey, vcnt, uid, logCap, vmask, capV, distinctVisited);
         }
         distinctVisited = (uint32_t)readlane((int)distinctVisited, 0);
         if (readlane((int)overflow, 0)) {
@@ -186,6 +237,14 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
                     v[r] = *reinterpret_cast<const int4*>(L.base + (size_t)id * kVecBytes + sub * 16);
                 }
             }
+            // expansion count of the candidate this lane ends up holding, read while the
+            // vectors are in flight
+            const uint32_t il = (uint32_t)sub * 8 + grp;
+            uint32_t cid = 0, cnt = 0;
+            if (il < nb) {
+                cid = pend[b * 64 + il];
+                cnt = visitedCount(vkey, vcnt, cid, logCap, vmask);
+            }
             int part8[8];
 #pragma unroll
             for (int r = 0; r < 8; r++) {
@@ -193,11 +252,9 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
                 part8[r] = slicePartial(v[r], qv);
             }
             const int tot = transposeReduce8(part8, lane);
-            const uint32_t il = (uint32_t)sub * 8 + grp;  // candidate held by this lane
             if (il < nb) {
                 const int d = qq + tot;
-                const uint32_t id = pend[b * 64 + il];
-                if (d < cutoff) key[b] = ((uint64_t)(uint32_t)d << 32) | id;
+                if (d < cutoff) key[b] = countKey(d, cid, cnt);
             }
         }
 
@@ -221,8 +278,9 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
         bool enter[MAXCH];
 #pragma unroll
         for (int b = 0; b < MAXCH; b++) {
-            const uint64_t kv = key[b];
-            const bool valid = kv != ~0ull;
+            const bool valid = key[b] != ~0ull;
+            const uint64_t kv = valid ? frontierOfCountKey(key[b]) : ~0ull;
+            const int cnt = (int)(key[b] & 0x1FFull);
             int rep = 0, lb = 0, cF = 0;
             if (valid) {
                 const int e = b * 64 + lane;
@@ -255,15 +313,6 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
             jPos[b] = lb + cF + (int)jK;
             if (enter[b]) {
                 // this copy is visited if the id was expanded more than `rep` times
-                const uint32_t id = (uint32_t)kv;
-                uint32_t h = visitedSlot(id, logCap);
-                int cnt = 0;
-                while (true) {
-                    const uint32_t kk = vkey[h];
-                    if (kk == id) { cnt = vcnt[h]; break; }
-                    if (kk == kEmpty) break;
-                    h = (h + 1) & vmask;
-                }
                 kkey[jK] = kv;
                 kvis[jK] = (unsigned char)(rep < cnt ? 1 : 0);
                 if (jPos[b] < Ls) occ[jPos[b]] = 1;
@@ -1090,9 +1139,10 @@ __device__ void searchFast2(const FlatLaunch& L, unsigned char* lds, uint32_t q,
 
 // FAST - 1, bit 0: tables below 4 GiB (32-bit offsets), bit 1: next-row prefetch, bit 2:
 // rows carry their squared norm: every query of the launch takes searchFast (Ls <= 64,
-// rows <= 64, layout flatFastLdsLayout).  FAST 0: queries with Ls <= 64 on rows of at most 64 entries take
-// it when the launch holds a filter of at least kFastFilterBits in LDS; the others take
-// searchOne.
+// rows <= 64, layout flatFastLdsLayout).  FAST 9..12: searchFast2.  FAST 0: queries with
+// Ls <= 64 on rows of at most 64 entries take searchFast when the launch holds a filter
+// of at least kFastFilterBits in LDS; the others take searchOne.  FAST -1: searchOne with
+// the visited table in HBM (L.gvkey / L.gvcnt, L.capV slots per wave).
 template <int MAXCH, int FAST>
 __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1102,6 +1152,9 @@ __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
     const uint32_t waveGlobal = blockIdx.x * wavesPerBlock + waveInBlock;
     unsigned char* lds = smem + (size_t)waveInBlock * L.ldsPerWave;
     uint32_t* gfilt = L.gfilt ? L.gfilt + ((size_t)waveGlobal << L.gbitsMax) : nullptr;
+    constexpr bool VG = FAST < 0;
+    uint32_t* gvk = VG ? L.gvkey + (size_t)waveGlobal * L.capV : nullptr;
+    unsigned char* gvc = VG ? L.gvcnt + (size_t)waveGlobal * L.capV : nullptr;
 
     Prof P;
     P.start();
@@ -1132,7 +1185,7 @@ __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
             } else {
                 searchFast2<((FAST - 9) & 1) != 0, ((FAST - 9) & 2) != 0>(L, lds, q, c, gfilt, P);
             }
-        } else if constexpr (FAST != 0) {
+        } else if constexpr (FAST > 0) {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride) {
                 if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
             } else {
@@ -1142,13 +1195,13 @@ __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride ||
                 (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
                 if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
-            } else if (MAXCH == 1 && c.Ls <= kFastLsMax && L.fbitsLds >= kFastFilterBits) {
+            } else if (!VG && MAXCH == 1 && c.Ls <= kFastLsMax && L.fbitsLds >= kFastFilterBits) {
                 if (L.normsInRow) searchFast<false, false, true>(L, lds, q, c, P);
                 else searchFast<false, false, false>(L, lds, q, c, P);
             } else if (fb <= L.fbitsLds) {
-                searchOne<MAXCH, false>(L, lds, q, c, nullptr, P);
+                searchOne<MAXCH, false, VG>(L, lds, q, c, nullptr, gvk, gvc, P);
             } else {
-                searchOne<MAXCH, true>(L, lds, q, c, gfilt, P);
+                searchOne<MAXCH, true, VG>(L, lds, q, c, gfilt, gvk, gvc, P);
             }
         }
         q = L.queue ? takeQuery(L.queue, L.order, L.nq, lane) : q + L.totalWaves;
@@ -1160,6 +1213,15 @@ __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
 template <int MAXCH>
 __global__ __launch_bounds__(256) void beamFlatKernel(FlatLaunch L) {
     drainQueries<MAXCH, 0>(L);
+}
+
+// Long beams: with the visited table in HBM a wavefront needs only its frontier and a few
+// small buffers in LDS, so up to four wavefronts share a SIMD (128 VGPRs; rows of more
+// than 64 entries need more and get three); each hop is a chain of dependent HBM round
+// trips and the wavefronts hide one another's.
+template <int MAXCH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MAXCH == 1 ? 4 : 3, 8))) void beamFlatWideKernel(FlatLaunch L) {
+    drainQueries<MAXCH, -1>(L);
 }
 
 // The register-resident kernel is held to 96 VGPRs so five wavefronts share a SIMD (its
@@ -1202,6 +1264,16 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
                           hipStream_t stream) {
     const size_t lds = (size_t)L.ldsPerWave * wavesPerBlock;
     const dim3 grid(blocks), block(64 * wavesPerBlock);
+    if (L.fast < 0) {
+        if (L.gvkey == nullptr || L.gvcnt == nullptr) return hipErrorInvalidValue;
+        switch (maxch) {
+            case 1: hipLaunchKernelGGL((beamFlatWideKernel<1>), grid, block, lds, stream, L); break;
+            case 2: hipLaunchKernelGGL((beamFlatWideKernel<2>), grid, block, lds, stream, L); break;
+            case 4: hipLaunchKernelGGL((beamFlatWideKernel<4>), grid, block, lds, stream, L); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (L.fast) {
         if (maxch != 1) return hipErrorInvalidValue;
         if (L.fast >= 9) {
@@ -1242,6 +1314,14 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
 hipError_t beamFlatOccupancy(int maxch, int fast, int wavesPerBlock, size_t ldsBytes,
                              int* blocksPerCU) {
     const int threads = 64 * wavesPerBlock;
+    if (fast < 0) {
+        switch (maxch) {
+            case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatWideKernel<1>, threads, ldsBytes);
+            case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatWideKernel<2>, threads, ldsBytes);
+            case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatWideKernel<4>, threads, ldsBytes);
+            default: return hipErrorInvalidValue;
+        }
+    }
     if (fast >= 9) {
         if (maxch != 1) return hipErrorInvalidValue;
         return fast == 9 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFast2Kernel<9>, threads, ldsBytes)

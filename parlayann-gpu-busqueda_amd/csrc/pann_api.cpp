@@ -454,10 +454,11 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
     const bool fast = maxch == 1 && Ls <= kFastLsMax;             // one frontier key per lane
     const bool fast2 = maxch == 1 && !fast && Ls <= kFast2LsMax;  // two, filter in HBM
     const bool ldsFilter = fbits <= kFilterBitsLds && !fast2;
-    const int fastKind = fast ? fastMode(g->n, g->G) : fast2 ? 8 + fastMode(g->n, g->G) : 0;
+    const bool hbmVisited = !fast && !fast2 && capV >= kVisitedHbmMinCap;
+    const int fastKind = fast ? fastMode(g->n, g->G) : fast2 ? 8 + fastMode(g->n, g->G) : hbmVisited ? -1 : 0;
     const FlatLds lay = fast ? flatFastLdsLayout(capV)
                       : fast2 ? flatFast2LdsLayout(capV)
-                              : flatLdsLayout(Ls, maxch, capV, ldsFilter ? fbits : 0);
+                              : flatLdsLayout(Ls, maxch, hbmVisited ? 0 : capV, ldsFilter ? fbits : 0);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "Ls=%d needs %u bytes of LDS per query", Ls, lay.total);
 
@@ -503,6 +504,16 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
         }
     }
     L.gfilt = s->gfilt;
+    if (hbmVisited) {
+        const size_t slots = (size_t)totalWaves * capV;
+        he = hipMalloc(&s->gvis, slots * 5);
+        if (he != hipSuccess) {
+            pann_searcher_free(s);
+            return setError(PANN_ERR_HIP, "allocating visited tables: %s", hipGetErrorString(he));
+        }
+        L.gvkey = s->gvis;
+        L.gvcnt = reinterpret_cast<unsigned char*>(s->gvis + slots);
+    }
     r = allocOutputs(s->out, nq, (uint32_t)std::max(1, cfg->k));
     if (r != PANN_OK) {
         pann_searcher_free(s);
@@ -688,6 +699,7 @@ void pann_searcher_free(pann_searcher* s) {
     if (!s) return;
     freeOutputs(s->out);
     hipFree(s->gfilt);
+    hipFree(s->gvis);
     delete s;
 }
 

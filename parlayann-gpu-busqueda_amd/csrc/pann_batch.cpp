@@ -39,6 +39,7 @@ struct pann_batch {
     uint32_t* order = nullptr;
     uint32_t* queue = nullptr;
     uint32_t* gfilt = nullptr;
+    uint32_t* gvis = nullptr;  // HBM visited tables of the FlatLaunch::fast -1 launches
     pann_search_outputs out{};
     std::vector<uint32_t> hostOrder;
     hipGraph_t graphDef = nullptr;
@@ -99,6 +100,11 @@ int allocFilters(pann_batch* b, size_t bytes) {
     return PANN_OK;
 }
 
+int allocVisited(pann_batch* b, size_t bytes) {
+    if (bytes) HIP_TRY(hipMalloc(&b->gvis, bytes));
+    return PANN_OK;
+}
+
 // Wavefronts per block and resident blocks for one kernel of the batch.
 int sizeLaunch(pann_batch* b, int fastKind, uint32_t ldsPerWave, int* wpbOut, int* blocksOut) {
     int cus = 0;
@@ -130,12 +136,16 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, FlatPart* P, int* 
     // one frontier key per lane (filter in LDS), or two (64 < Ls <= 128, filter in HBM)
     const bool fast = b->maxch == 1 && LsHi <= kFastLsMax && fbLds >= kFastFilterBits;
     const bool fast2 = b->maxch == 1 && !fast && LsHi <= kFast2LsMax;
+    const bool hbmVisited = !fast && !fast2 && capV >= kVisitedHbmMinCap;
+    // a general launch whose shortest beam already needs a larger filter keeps none in LDS
+    const int fbLaunch = (!fast && !fast2 && flatFilterBits(LsLo + 1) > fbLds) ? 0 : fbLds;
     const int mode = fastMode(b->graph->n, b->graph->G);
     P->LsLo = LsLo;
     P->LsHi = LsHi;
-    P->fastKind = fast ? mode + (c.prefetch ? 2 : 0) : fast2 ? 8 + mode : 0;
+    P->fastKind = fast ? mode + (c.prefetch ? 2 : 0) : fast2 ? 8 + mode : hbmVisited ? -1 : 0;
     const FlatLds lay = fast ? flatFastLdsLayout(capV)
-                      : fast2 ? flatFast2LdsLayout(capV) : flatLdsLayout(LsHi, b->maxch, capV, fbLds);
+                      : fast2 ? flatFast2LdsLayout(capV)
+                              : flatLdsLayout(LsHi, b->maxch, hbmVisited ? 0 : capV, fbLaunch);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsHi, lay.total);
     int r = sizeLaunch(b, P->fastKind, lay.total, &P->wavesPerBlock, &P->blocks);
@@ -145,7 +155,7 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, FlatPart* P, int* 
     L.n = b->graph->n;
     L.G = b->graph->G;
     L.LsMax = LsHi;
-    L.fbitsLds = fbLds;
+    L.fbitsLds = fast || fast2 ? fbLds : fbLaunch;
     L.gbitsMax = *gbits;
     L.capV = capV;
     L.ldsPerWave = lay.total;
@@ -186,7 +196,7 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
         if (hi < LsMax) his.push_back(hi);
     his.push_back(LsMax);
     b->parts.resize(his.size());
-    size_t gfiltBytes = 0;
+    size_t gfiltBytes = 0, gvisBytes = 0;
     for (size_t i = 0; i < his.size(); i++) {
         int gbits = 0;
         r = planFlatPart(b, i ? his[i - 1] : 0, his[i], i + 1 == his.size(), &b->parts[i], &gbits);
@@ -198,6 +208,7 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
         const uint32_t waves = (uint32_t)P.blocks * P.wavesPerBlock;
         b->totalWaves = std::max(b->totalWaves, waves);
         if (gbits) gfiltBytes = std::max(gfiltBytes, ((size_t)waves << gbits) * 4);
+        if (P.fastKind < 0) gvisBytes = std::max(gvisBytes, (size_t)waves * P.L.capV * 5);
     }
     const FlatPart& top = b->parts.back();
     b->wavesPerBlock = top.wavesPerBlock;
@@ -205,6 +216,7 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
     b->cfg.waves_per_block = top.wavesPerBlock;
     // the launches run one after another on one stream and share the global filters
     r = allocFilters(b, gfiltBytes);
+    if (r == PANN_OK) r = allocVisited(b, gvisBytes);
     if (r == PANN_OK) r = allocCommon(b);
     if (r != PANN_OK) {
         pann_batch_free(b);
@@ -213,6 +225,11 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
     for (FlatPart& P : b->parts) {
         FlatLaunch& L = P.L;
         L.gfilt = b->gfilt;
+        if (P.fastKind < 0) {
+            const size_t slots = (size_t)P.blocks * P.wavesPerBlock * L.capV;
+            L.gvkey = b->gvis;
+            L.gvcnt = reinterpret_cast<unsigned char*>(b->gvis + slots);
+        }
         L.qp = b->params;
         L.qids = nullptr;
         L.queries = b->queries;
@@ -448,6 +465,7 @@ void pann_batch_free(pann_batch* b) {
     hipFree(b->order);
     hipFree(b->queue);
     hipFree(b->gfilt);
+    hipFree(b->gvis);
     freeOutputs(b->out);
     delete b;
 }

@@ -126,6 +126,9 @@ __host__ __device__ inline int flatFilterBits(int Ls) {
 __host__ __device__ inline uint32_t flatVisitedCap(int Ls) {
     return 1u << ceilLog2((uint64_t)8 * Ls + 128);
 }
+// General-form launches whose visited table has at least this many slots keep it in HBM
+// (FlatLaunch::fast -1): 5 bytes a slot would otherwise hold LDS (and so occupancy) down.
+constexpr uint32_t kVisitedHbmMinCap = 2048;
 
 struct FlatLaunch {
     const int8_t* base;        // n rows, kVecBytes pitch, zero padded
@@ -149,8 +152,11 @@ struct FlatLaunch {
                                // 2 when base and adjacency are below 4 GiB; +2 with next-row
                                // prefetch (the launcher adds 4 for normsInRow).  Two-register
                                // form (flatFast2LdsLayout, filter in gfilt): 9, or 10 below
-                               // 4 GiB (the launcher adds 2 for normsInRow).  0: general
+                               // 4 GiB (the launcher adds 2 for normsInRow).  0: general;
+                               // -1: general with the visited table in HBM (gvkey/gvcnt)
     uint32_t* gfilt;           // per-wave global filter regions
+    uint32_t* gvkey;           // fast -1: capV visited keys per wave (then the counts:
+    unsigned char* gvcnt;      //          capV bytes per wave)
     uint32_t* queue;           // persistent mode: {next ticket, finished waves}; null: static
     const uint32_t* order;     // persistent mode: ticket -> query (null: identity)
     uint32_t totalWaves;

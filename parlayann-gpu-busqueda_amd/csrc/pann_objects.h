@@ -46,6 +46,7 @@ struct pann_searcher {
     FlatLaunch L{};
     HnswLaunch H{};
     uint32_t* gfilt = nullptr;
+    uint32_t* gvis = nullptr;  // HBM visited tables (FlatLaunch::fast -1)
     pann_search_outputs out{};
 };
 

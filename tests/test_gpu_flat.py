@@ -38,7 +38,7 @@ def _assert_same(r, o, k, tag):
         f"flags {o.flags[bad[0]]} hops {o.hops[bad[0]]}")
 
 
-@pytest.mark.parametrize("Ls", [10, 20, 40])
+@pytest.mark.parametrize("Ls", [10, 20, 40, 300])
 def test_w1_parity(gpu, w1, Ls):
     base, queries, graph = w1
     r = _run(base, graph, queries, Ls, 10)
@@ -82,7 +82,7 @@ def test_duplicate_edges_and_wide_rows(gpu, max_degree, dup):
     n = 3000
     base, queries = datagen.gmm_dataset(n, 150, dim=100, ncomp=6, seeds=(21, 22, 23))
     graph = random_graph(n, max_degree, rng, dup_frac=dup, min_degree=max(1, max_degree // 3))
-    for Ls in (10, 33, 90):
+    for Ls in (10, 33, 90, 150):
         r = _run(base, graph, queries, Ls, 10)
         o = _oracle(base, graph, queries, Ls, 10)
         _assert_same(r, o, 10, f"G={max_degree} dup={dup} Ls={Ls}")
@@ -153,7 +153,7 @@ def test_extreme_distances(gpu, dim):
     base = np.where(rng.random((n, dim)) < 0.9, -128, 127).astype(np.int8)
     queries = np.where(rng.random((120, dim)) < 0.9, 127, -128).astype(np.int8)
     graph = random_graph(n, 48, rng, dup_frac=0.05, min_degree=16)
-    for Ls in (10, 40, 100):
+    for Ls in (10, 40, 100, 200):
         r = _run(base, graph, queries, Ls, 10)
         o = _oracle(base, graph, queries, Ls, 10)
         assert int(o.dists[:, :10].max()) > (1 << 22)
