@@ -394,7 +394,9 @@ __device__ __forceinline__ bool seenFilterEnd(uint32_t* filt, const SeenProbe& s
         cm &= ~M;
     }
     const bool seen = part && (prev == a);
-    if (part && last) filtStore<G>(filt, h, a);
+    // an HBM slot that already holds a is not written again (most entries of a long
+    // beam's rows were seen before, and each write is a partial-line HBM write)
+    if (part && last && (!G || s.prev != a)) filtStore<G>(filt, h, a);
     if constexpr (G) __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
     return part && !seen;
