@@ -114,9 +114,9 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
     const int grp = lane >> 3, sub = lane & 7;
     const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
-    uint64_t* fkey[2] = {reinterpret_cast<uint64_t*>(lds + L.lds.fkey[0]),
-                         reinterpret_cast<uint64_t*>(lds + L.lds.fkey[1])};
-    unsigned char* fvis[2] = {lds + L.lds.fvis[0], lds + L.lds.fvis[1]};
+    // the frontier (keys and visited flags), merged in place
+    uint64_t* F = reinterpret_cast<uint64_t*>(lds + L.lds.fkey);
+    unsigned char* FV = lds + L.lds.fvis;
     unsigned char* occ = lds + L.lds.occ;
     uint64_t* kkey = reinterpret_cast<uint64_t*>(lds + L.lds.kkey);
     unsigned char* kvis = lds + L.lds.kvis;
@@ -155,12 +155,11 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
     const int d0 = groupDistance(L.base, 0u, sub, qv, qq, normMask);
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) {
-        fkey[0][0] = (uint64_t)(uint32_t)d0 << 32;
-        fvis[0][0] = 0;
+        F[0] = (uint64_t)(uint32_t)d0 << 32;
+        FV[0] = 0;
     }
     __builtin_amdgcn_wave_barrier();
 
-    int cur = 0;
     int fsize = 1;
     int u = 0;
     int64_t visits = 0;
@@ -172,7 +171,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
     P.count(kProfQueries);
 
     while (u >= 0 && visits < c.limit) {
-        const uint64_t ukey = fkey[cur][u];
+        const uint64_t ukey = F[u];
         const uint32_t uid = (uint32_t)ukey;
 
         // ---- adjacency row, one coalesced dword per lane per 64 entries ----
@@ -187,7 +186,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
         // ---- visit u: frontier flag and visited multiset count ----
         bool overflow = false;
         if (lane == 0) {
-            fvis[cur][u] = 1;
+            FV[u] = 1;
             overflow = !visitedInsert(vkey, vcnt, uid, logCap, vmask, capV, distinctVisited);
         }
         distinctVisited = (uint32_t)readlane((int)distinctVisited, 0);
@@ -200,7 +199,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
         P.mark(kProfVisit);
         P.count(kProfHops);
 
-        const int cutoff = (fsize < Ls) ? 0x7FFFFFFF : (int)(fkey[cur][fsize - 1] >> 32);
+        const int cutoff = (fsize < Ls) ? 0x7FFFFFFF : (int)(F[fsize - 1] >> 32);
 
         // ---- lossy seen filter in row order (ParlayANN has_been_seen) ----
         uint32_t nP = 0;
@@ -264,7 +263,6 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
         P.mark(kProfSort);
 
         // ---- multiset union with the frontier ----
-        const uint64_t* F = fkey[cur];
         uint64_t runStarts[MAXCH];
 #pragma unroll
         for (int b = 0; b < MAXCH; b++) {
@@ -274,6 +272,8 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
             runStarts[b] = ballot(up != key[b]);
         }
         uint32_t kBase = 0;
+        int placed = 0;  // entering copies that land inside the first Ls positions
+        int pmin = -1;   // merged position of the first entering copy
         int jPos[MAXCH];
         bool enter[MAXCH];
 #pragma unroll
@@ -311,6 +311,8 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
             const uint32_t jK = kBase + mbcnt(em);
             kBase += (uint32_t)__popcll(em);
             jPos[b] = lb + cF + (int)jK;
+            if (pmin < 0 && em) pmin = readlane(jPos[b], __ffsll((unsigned long long)em) - 1);
+            placed += __popcll(ballot(enter[b] && jPos[b] < Ls));
             if (enter[b]) {
                 // this copy is visited if the id was expanded more than `rep` times
                 kkey[jK] = kv;
@@ -322,67 +324,80 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
         __builtin_amdgcn_wave_barrier();
 
         P.mark(kProfUnion);
-        // ---- merged frontier: each position finds its source by rank ----
-        const int nxt = cur ^ 1;
-        const int newSize = min(Ls, fsize + nK);
-        {
-            uint32_t rankBase = 0;
-            for (int p0 = 0; p0 < newSize; p0 += 64) {
+        // ---- merge in place: positions from the first entering copy on, last chunk first,
+        // each position taking its source by rank (sources never lie above it) ----
+        const int uPrev = u;
+        if (placed > 0) {
+            const int newSize = min(Ls, fsize + nK);
+            int rankEnd = placed;  // entering copies before the current chunk's end
+            for (int p0 = pmin + ((newSize - pmin - 1) & ~63); p0 >= pmin; p0 -= 64) {
                 const int p = p0 + lane;
                 const bool inb = p < newSize;
                 const bool o = inb && occ[p];
                 const uint64_t ob = ballot(o);
-                const uint32_t rank = rankBase + mbcnt(ob);
+                const int rank = rankEnd - __popcll(ob) + (int)mbcnt(ob);
+                uint64_t nk = 0;
+                unsigned char nv = 0;
                 if (inb) {
-                    uint64_t nk;
-                    unsigned char nv;
                     if (o) {
                         nk = kkey[rank];
                         nv = kvis[rank];
                     } else {
                         nk = F[p - rank];
-                        nv = fvis[cur][p - rank];
+                        nv = FV[p - rank];
                     }
-                    fkey[nxt][p] = nk;
-                    fvis[nxt][p] = nv;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (inb) {
+                    F[p] = nk;
+                    FV[p] = nv;
                     occ[p] = 0;
                 }
-                rankBase += (uint32_t)__popcll(ob);
+                rankEnd -= __popcll(ob);
             }
+            __builtin_amdgcn_wave_barrier();
+            fsize = newSize;
         }
-        __builtin_amdgcn_wave_barrier();
-        cur = nxt;
-        fsize = newSize;
 
         P.mark(kProfMerge);
-        // ---- cut: upper_bound(F, (0, (float)(cut * (double)F[k].d))) ----
-        if (c.k > 0 && fsize > c.k) {
-            const int dk = (int)(fkey[cur][c.k] >> 32);
+        // ---- cut: upper_bound(F, (0, (float)(cut * (double)F[k].d))).  The predicate is
+        // monotone over the sorted frontier: lane l tests the last position of stretch l
+        // (stretches of ceil(fsize / 64)), the first true lane names the stretch where it
+        // turns true, a second ballot the position.  Nothing entered: nothing to cut ----
+        if (placed > 0 && c.k > 0 && fsize > c.k) {
+            const int dk = (int)(F[c.k] >> 32);
             const float t = (float)(c.cut * (double)dk);
-            int nf = fsize;
-            for (int p0 = 0; p0 < fsize; p0 += 64) {
-                const int p = p0 + lane;
-                bool pred = false;
-                if (p < fsize) {
-                    const uint64_t kv = fkey[cur][p];
-                    const float df = (float)(int)(kv >> 32);
-                    pred = (t < df) || (t == df && (uint32_t)kv > 0u);
+            const int stride = (fsize + 63) >> 6;
+            auto pred = [&](int p) {
+                const uint64_t kv = F[p];
+                const float df = (float)(int)(kv >> 32);
+                return (t < df) || (t == df && (uint32_t)kv > 0u);
+            };
+            const int lo0 = lane * stride;
+            const uint64_t m = ballot(lo0 < fsize && pred(min(lo0 + stride, fsize) - 1));
+            if (m) {
+                const int f = __ffsll((unsigned long long)m) - 1;
+                const int lo = f * stride, hi = min(lo + stride, fsize);
+                int nf = hi - 1;  // the predicate holds there
+                for (int p0 = lo; p0 < hi; p0 += 64) {
+                    const int p = p0 + lane;
+                    const uint64_t m2 = ballot(p < hi && pred(p));
+                    if (m2) {
+                        nf = p0 + __ffsll((unsigned long long)m2) - 1;
+                        break;
+                    }
                 }
-                const uint64_t m = ballot(pred);
-                if (m) {
-                    nf = p0 + __ffsll((unsigned long long)m) - 1;
-                    break;
-                }
+                fsize = nf;
             }
-            fsize = nf;
         }
 
         P.mark(kProfCut);
-        // ---- next: first frontier copy not visited ----
+        // ---- next: first frontier copy not visited.  Positions below both the first
+        // entering copy and u + 1 hold old, visited copies ----
         u = -1;
-        for (int p0 = 0; p0 < fsize; p0 += 64) {
+        for (int p0 = placed > 0 ? min(pmin, uPrev + 1) : uPrev + 1; p0 < fsize; p0 += 64) {
             const int p = p0 + lane;
-            const bool nv = (p < fsize) && !fvis[cur][p];
+            const bool nv = (p < fsize) && !FV[p];
             const uint64_t m = ballot(nv);
             if (m) {
                 u = p0 + __ffsll((unsigned long long)m) - 1;
@@ -398,7 +413,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
     int32_t* od = L.outDists + (size_t)q * L.outStride;
     for (int j = lane; j < c.k; j += 64) {
         if (j < fsize) {
-            const uint64_t kv = fkey[cur][j];
+            const uint64_t kv = F[j];
             oi[j] = (uint32_t)kv;
             od[j] = (int32_t)(kv >> 32);
         } else {

@@ -31,8 +31,8 @@ static_assert(sizeof(QueryParams) == 32, "QueryParams layout is part of the C AB
 
 // Byte offsets of the per-wave LDS regions of the flat search.
 struct FlatLds {
-    uint32_t fkey[2];  // frontier keys (d << 32 | id), two buffers of LsMax
-    uint32_t fvis[2];  // frontier visited flags (u8), two buffers
+    uint32_t fkey;     // frontier keys (d << 32 | id), LsMax
+    uint32_t fvis;     // frontier visited flags (u8), LsMax
     uint32_t occ;      // u8 per merged position
     uint32_t kkey;     // entering candidate keys (64*maxch)
     uint32_t kvis;     // u8
@@ -51,14 +51,12 @@ __host__ __device__ inline FlatLds flatLdsLayout(int LsMax, int maxch, uint32_t 
     FlatLds L;
     uint32_t o = 0;
     const uint32_t cmax = 64u * (uint32_t)maxch;
-    L.fkey[0] = o; o += 8u * LsMax;
-    L.fkey[1] = o; o += 8u * LsMax;
+    L.fkey = o;    o += 8u * LsMax;
     L.kkey = o;    o += 8u * cmax;
     L.pend = o;    o += 4u * cmax;
     L.vkey = o;    o += 4u * capV;
     L.filt = o;    o += (fbitsLds > 0) ? (4u << fbitsLds) : 0u;
-    L.fvis[0] = o; o += alignUp(LsMax, 8);
-    L.fvis[1] = o; o += alignUp(LsMax, 8);
+    L.fvis = o;    o += alignUp(LsMax, 8);
     L.occ = o;     o += alignUp(LsMax, 8);
     L.kvis = o;    o += alignUp(cmax, 8);
     L.tag = o;     o += kTagSlots;
