@@ -25,12 +25,23 @@ __device__ __forceinline__ uint32_t visitedSlot(uint32_t id, uint32_t logCap) {
 }
 
 // Visited multiset of the general form: capV id slots (kEmpty: free) and their expansion
-// counts (u8), in LDS or in a per-wave HBM region.  An id hashes to an aligned group of
-// four slots and takes the first free slot from there on, group by group, so one 16-B key
-// load and one 4-B count load answer most lookups.  Inserts stop one short of capV, so
-// every walk meets a free slot.
+// counts (u8), in LDS or in a per-wave HBM region.  In LDS (GROUPS false) an id takes the
+// first free slot from its hash slot on, one probe at a time.  In HBM (GROUPS true) it
+// hashes to an aligned group of four slots and takes the first free slot from there on,
+// group by group, so one 16-B key load and one 4-B count load answer most lookups.
+// Inserts stop one short of capV, so every walk meets a free slot.
+template <bool GROUPS>
 __device__ __forceinline__ uint32_t visitedCount(const uint32_t* vkey, const unsigned char* vcnt,
                                                  uint32_t id, uint32_t logCap, uint32_t vmask) {
+    if constexpr (!GROUPS) {
+        uint32_t h = visitedSlot(id, logCap);
+        while (true) {
+            const uint32_t k = vkey[h];
+            if (k == id) return vcnt[h];
+            if (k == kEmpty) return 0;
+            h = (h + 1) & vmask;
+        }
+    }
     uint32_t g = visitedSlot(id, logCap) & ~3u;
     while (true) {
         const uint4 k4 = *reinterpret_cast<const uint4*>(vkey + g);
@@ -46,9 +57,23 @@ __device__ __forceinline__ uint32_t visitedCount(const uint32_t* vkey, const uns
 }
 
 // One more expansion of id (one lane).  Returns false when a new id would fill the table.
+template <bool GROUPS>
 __device__ __forceinline__ bool visitedInsert(uint32_t* vkey, unsigned char* vcnt, uint32_t id,
                                               uint32_t logCap, uint32_t vmask, uint32_t capV,
                                               uint32_t& distinct) {
+    if constexpr (!GROUPS) {
+        uint32_t h = visitedSlot(id, logCap);
+        while (vkey[h] != kEmpty && vkey[h] != id) h = (h + 1) & vmask;
+        if (vkey[h] == id) {
+            vcnt[h] = (unsigned char)(vcnt[h] + 1);
+            return true;
+        }
+        if (distinct + 1 >= capV) return false;
+        vkey[h] = id;
+        vcnt[h] = 1;
+        distinct++;
+        return true;
+    }
     uint32_t g = visitedSlot(id, logCap) & ~3u;
     while (true) {
         const uint4 k4 = *reinterpret_cast<const uint4*>(vkey + g);
@@ -187,7 +212,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
         bool overflow = false;
         if (lane == 0) {
             FV[u] = 1;
-            overflow = !visitedInsert(vkey, vcnt, uid, logCap, vmask, capV, distinctVisited);
+            overflow = !visitedInsert<VGLOBAL>(vkey, vcnt, uid, logCap, vmask, capV, distinctVisited);
         }
         distinctVisited = (uint32_t)readlane((int)distinctVisited, 0);
         if (readlane((int)overflow, 0)) {
@@ -242,7 +267,7 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
             uint32_t cid = 0, cnt = 0;
             if (il < nb) {
                 cid = pend[b * 64 + il];
-                cnt = visitedCount(vkey, vcnt, cid, logCap, vmask);
+                cnt = visitedCount<VGLOBAL>(vkey, vcnt, cid, logCap, vmask);
             }
             int part8[8];
 #pragma unroll

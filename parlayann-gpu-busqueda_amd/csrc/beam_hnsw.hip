@@ -11,8 +11,9 @@
 //   * the next layer starts from W's farthest element; output W[0:k] as base point ids.
 // Layout choices of this file: W and cand live in registers as wave-wide sorted lists
 // (element e = r*64 + lane), so an insertion position is one ballot popcount and an
-// insertion a lane shift; is_inw is an LDS open-addressing hash; the sequential
-// acceptance only visits neighbours that can still pass the threshold, found by ballot.
+// insertion a lane shift; is_inw is an open-addressing hash in LDS or, for large ef, in a
+// per-wave HBM region; the sequential acceptance only visits neighbours that can still
+// pass the threshold, found by ballot.
 #include "device_common.h"
 
 namespace pann {
@@ -75,20 +76,104 @@ __device__ __forceinline__ uint32_t inwSlot(uint32_t id, uint32_t logCap) {
     return (id * 2654435761u) >> (32 - logCap);
 }
 
+// is_inw, open addressing.  In LDS (GROUPS false) an id takes the first free slot from its
+// hash slot on, one 4-B probe at a time.  In HBM (GROUPS true) it hashes to an aligned
+// group of four slots and takes the first free slot from there on, group by group, so one
+// 16-B load answers most lookups.  The layer search stops before the table fills, so every
+// walk meets a free slot.
+template <bool GROUPS>
+__device__ __forceinline__ bool inwContains(const uint32_t* inw, uint32_t id, uint32_t logCap,
+                                            uint32_t mask) {
+    if constexpr (!GROUPS) {
+        uint32_t h = inwSlot(id, logCap);
+        while (true) {
+            const uint32_t x = inw[h];
+            if (x == id) return true;
+            if (x == kEmpty) return false;
+            h = (h + 1) & mask;
+        }
+    }
+    uint32_t g = inwSlot(id, logCap) & ~3u;
+    while (true) {
+        const uint4 k4 = *reinterpret_cast<const uint4*>(inw + g);
+        const uint32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (kk[j] == id) return true;
+            if (kk[j] == kEmpty) return false;
+        }
+        g = (g + 4) & mask;
+    }
+}
+
+// Adds id (one lane, nothing else writing): true if it was new.
+template <bool GROUPS>
+__device__ __forceinline__ bool inwInsertOne(uint32_t* inw, uint32_t id, uint32_t logCap, uint32_t mask) {
+    if constexpr (!GROUPS) {
+        uint32_t h = inwSlot(id, logCap);
+        while (inw[h] != kEmpty && inw[h] != id) h = (h + 1) & mask;
+        if (inw[h] == id) return false;
+        inw[h] = id;
+        return true;
+    }
+    uint32_t g = inwSlot(id, logCap) & ~3u;
+    while (true) {
+        const uint4 k4 = *reinterpret_cast<const uint4*>(inw + g);
+        const uint32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (kk[j] == id) return false;
+            if (kk[j] == kEmpty) {
+                inw[g + j] = id;
+                return true;
+            }
+        }
+        g = (g + 4) & mask;
+    }
+}
+
+// Adds id, not yet present, while other lanes add other ids: a slot is claimed by
+// compare-and-swap, and a lost race moves on to the next free slot.
+template <bool GROUPS>
+__device__ __forceinline__ void inwInsertShared(uint32_t* inw, uint32_t id, uint32_t logCap, uint32_t mask) {
+    if constexpr (!GROUPS) {
+        uint32_t h = inwSlot(id, logCap);
+        while (true) {
+            const uint32_t old = atomicCAS(&inw[h], kEmpty, id);
+            if (old == kEmpty || old == id) return;
+            h = (h + 1) & mask;
+        }
+    }
+    uint32_t g = inwSlot(id, logCap) & ~3u;
+    while (true) {
+        const uint4 k4 = *reinterpret_cast<const uint4*>(inw + g);
+        const uint32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+        int j = 0;
+        while (j < 4 && kk[j] != kEmpty) j++;
+        if (j == 4) {
+            g = (g + 4) & mask;
+            continue;
+        }
+        if (atomicCAS(&inw[g + j], kEmpty, id) == kEmpty) return;
+        // another lane took that slot: read the group again
+    }
+}
+
 struct LayerState {
     int comps, edges, hops, flags;
     uint32_t inwCount;
 };
 
 // Expands node u on `layer`: row gather, seen filter, distances, in-order acceptance.
-template <int WR, int MAXCH, bool G>
+// IG: is_inw in HBM, so membership is read while the vectors are in flight (in LDS it is
+// read after the distances, which keeps fewer registers live across the loads).
+template <int WR, int MAXCH, bool G, bool IG>
 __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* filt, uint32_t fmask,
                            uint32_t u, int layer, int ef, const int4& qv, int qq,
-                           WaveList<WR>& W, WaveList<WR>& cand, LayerState& st, uint32_t capInw,
-                           uint32_t logCap, int lane, uint64_t lanesBelow) {
+                           WaveList<WR>& W, WaveList<WR>& cand, LayerState& st, uint32_t* inw,
+                           uint32_t capInw, uint32_t logCap, int lane, uint64_t lanesBelow) {
     const int grp = lane >> 3, sub = lane & 7;
     uint32_t* pend = reinterpret_cast<uint32_t*>(lds + L.lds.pend);
-    uint32_t* inw = reinterpret_cast<uint32_t*>(lds + L.lds.inw);
     unsigned char* tag = lds + L.lds.tag;
     const uint32_t width = layer > 0 ? L.GA : L.G0;
     const uint32_t* row = layer > 0 ? L.adjUp + ((size_t)L.upStart[u] + layer - 1) * L.GA
@@ -136,6 +221,11 @@ __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* fi
             if ((uint32_t)r < rounds && i < nb)
                 v[r] = *reinterpret_cast<const int4*>(L.vec + (size_t)pend[b * 64 + i] * kVecBytes + sub * 16);
         }
+        // membership before this row's acceptances
+        if (IG && (uint32_t)lane < nb) {
+            id[b] = pend[b * 64 + lane];
+            memb[b] = inwContains<IG>(inw, id[b], logCap, capInw - 1);
+        }
         int part8[8];
         const int normMask = (L.normsInRow && sub == 7) ? -1 : 0;
 #pragma unroll
@@ -149,14 +239,9 @@ __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* fi
         if ((uint32_t)lane < nb) {
             valid[b] = true;
             dist[b] = qq + mine;
-            id[b] = pend[b * 64 + lane];
-            // membership before this row's acceptances
-            uint32_t h = inwSlot(id[b], logCap);
-            while (true) {
-                const uint32_t x = inw[h];
-                if (x == id[b]) { memb[b] = true; break; }
-                if (x == kEmpty) break;
-                h = (h + 1) & (capInw - 1);
+            if (!IG) {
+                id[b] = pend[b * 64 + lane];
+                memb[b] = inwContains<IG>(inw, id[b], logCap, capInw - 1);
             }
         }
     }
@@ -202,26 +287,20 @@ __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* fi
 #pragma unroll
     for (int b = 0; b < MAXCH; b++) {
         added += (uint32_t)__popcll(ballot(acc[b]));
-        if (acc[b]) {
-            uint32_t h = inwSlot(id[b], logCap);
-            while (true) {
-                const uint32_t old = atomicCAS(&inw[h], kEmpty, id[b]);
-                if (old == kEmpty || old == id[b]) break;
-                h = (h + 1) & (capInw - 1);
-            }
-        }
+        if (acc[b]) inwInsertShared<IG>(inw, id[b], logCap, capInw - 1);
     }
     st.inwCount += added;
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int WR, int MAXCH, bool FGLOBAL>
-__device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt, uint32_t q,
-                        int Ls, int k, int64_t visit) {
+// IG: is_inw in this wave's HBM region (ginw) rather than in LDS.
+template <int WR, int MAXCH, bool FGLOBAL, bool IG>
+__device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt, uint32_t* ginw,
+                        uint32_t q, int Ls, int k, int64_t visit) {
     const int lane = laneId();
     const int sub = lane & 7;
     const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint32_t* inw = reinterpret_cast<uint32_t*>(lds + L.lds.inw);
+    uint32_t* inw = IG ? ginw : reinterpret_cast<uint32_t*>(lds + L.lds.inw);
     uint32_t* ldsFilt = reinterpret_cast<uint32_t*>(lds + L.lds.filt);
     // rows may hold |p|^2 in bytes 124..127 (lane 7's .w): masked out of the products
     const int normMask = (L.normsInRow && sub == 7) ? -1 : 0;
@@ -252,7 +331,11 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
         uint32_t* filt = globalFilter ? gfilt : ldsFilt;
         if (globalFilter) filtClear<true>(filt, bits, lane);
         else filtClear<false>(filt, bits, lane);
-        for (uint32_t i = lane; i < capInw; i += 64) inw[i] = kEmpty;
+        {
+            const uint4 e4 = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+            uint4* i4 = reinterpret_cast<uint4*>(inw);
+            for (uint32_t i = lane; i < capInw / 4; i += 64) i4[i] = e4;
+        }
         st.inwCount = 0;
         W.clear();
         cand.clear();
@@ -264,12 +347,7 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
                 const uint32_t slot = (uint32_t)hash64(pe) & fmask;
                 if (globalFilter) filtStore<true>(filt, slot, pe);
                 else filtStore<false>(filt, slot, pe);
-                uint32_t h = inwSlot(pe, logCap);
-                while (inw[h] != kEmpty && inw[h] != pe) h = (h + 1) & (capInw - 1);
-                if (inw[h] == kEmpty) {
-                    inw[h] = pe;
-                    st.inwCount++;
-                }
+                if (inwInsertOne<IG>(inw, pe, logCap, capInw - 1)) st.inwCount++;
             }
             if (globalFilter) __builtin_amdgcn_s_waitcnt(0);
             __builtin_amdgcn_wave_barrier();
@@ -291,11 +369,11 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
             cand.popFront(lane);
             st.hops++;
             if (globalFilter)
-                expandNode<WR, MAXCH, true>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq, W,
-                                            cand, st, capInw, logCap, lane, lanesBelow);
+                expandNode<WR, MAXCH, true, IG>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq, W,
+                                            cand, st, inw, capInw, logCap, lane, lanesBelow);
             else
-                expandNode<WR, MAXCH, false>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq,
-                                             W, cand, st, capInw, logCap, lane, lanesBelow);
+                expandNode<WR, MAXCH, false, IG>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq,
+                                             W, cand, st, inw, capInw, logCap, lane, lanesBelow);
             if (st.inwCount + 64u * MAXCH >= capInw) {
                 st.flags |= kFlagOverflow;
                 break;
@@ -337,7 +415,7 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int WR, int MAXCH>
+template <int WR, int MAXCH, bool IG>
 __global__ __launch_bounds__(256) void beamHnswKernel(HnswLaunch L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = laneId();
@@ -346,6 +424,7 @@ __global__ __launch_bounds__(256) void beamHnswKernel(HnswLaunch L) {
     const uint32_t waveGlobal = blockIdx.x * wavesPerBlock + waveInBlock;
     unsigned char* lds = smem + (size_t)waveInBlock * L.ldsPerWave;
     uint32_t* gfilt = L.gfilt ? L.gfilt + ((size_t)waveGlobal << L.gbitsMax) : nullptr;
+    uint32_t* ginw = IG ? L.ginw + (size_t)waveGlobal * L.capInw : nullptr;
 
     // queue mode ends on kQueueEmpty, strided mode past the last query
     const uint32_t qEnd = L.queue ? kQueueEmpty : L.nq;
@@ -363,9 +442,9 @@ __global__ __launch_bounds__(256) void beamHnswKernel(HnswLaunch L) {
             (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
             if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
         } else if (fb <= L.fbitsLds) {
-            hnswOne<WR, MAXCH, false>(L, lds, nullptr, q, Ls, k, visit);
+            hnswOne<WR, MAXCH, false, IG>(L, lds, nullptr, ginw, q, Ls, k, visit);
         } else {
-            hnswOne<WR, MAXCH, true>(L, lds, gfilt, q, Ls, k, visit);
+            hnswOne<WR, MAXCH, true, IG>(L, lds, gfilt, ginw, q, Ls, k, visit);
         }
         q = L.queue ? takeQuery(L.queue, L.order, L.nq, lane) : q + L.totalWaves;
     }
@@ -374,14 +453,28 @@ __global__ __launch_bounds__(256) void beamHnswKernel(HnswLaunch L) {
 
 template <int WR, int MAXCH>
 hipError_t launchT(const HnswLaunch& L, int wpb, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL((beamHnswKernel<WR, MAXCH>), dim3(blocks), dim3(64 * wpb),
-                       (size_t)L.ldsPerWave * wpb, s, L);
+    const size_t lds = (size_t)L.ldsPerWave * wpb;
+    if (L.ginw) {
+        if constexpr (WR >= 2) {
+            hipLaunchKernelGGL((beamHnswKernel<WR, MAXCH, true>), dim3(blocks), dim3(64 * wpb), lds, s, L);
+        } else {
+            return hipErrorInvalidValue;
+        }
+    } else {
+        hipLaunchKernelGGL((beamHnswKernel<WR, MAXCH, false>), dim3(blocks), dim3(64 * wpb), lds, s, L);
+    }
     return hipGetLastError();
 }
 
 template <int WR, int MAXCH>
-hipError_t occT(int wpb, size_t lds, int* bpc) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, beamHnswKernel<WR, MAXCH>, 64 * wpb, lds);
+hipError_t occT(bool inwHbm, int wpb, size_t lds, int* bpc) {
+    if (inwHbm) {
+        if constexpr (WR >= 2)
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, beamHnswKernel<WR, MAXCH, true>, 64 * wpb, lds);
+        else
+            return hipErrorInvalidValue;
+    }
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, beamHnswKernel<WR, MAXCH, false>, 64 * wpb, lds);
 }
 
 }  // namespace
@@ -408,8 +501,8 @@ hipError_t launchBeamHnsw(const HnswLaunch& L, int wr, int maxch, int wavesPerBl
     PANN_HNSW_DISPATCH(launchT, L, wavesPerBlock, blocks, stream)
 }
 
-hipError_t beamHnswOccupancy(int wr, int maxch, int wavesPerBlock, size_t ldsBytes, int* bpc) {
-    PANN_HNSW_DISPATCH(occT, wavesPerBlock, ldsBytes, bpc)
+hipError_t beamHnswOccupancy(int wr, int maxch, bool inwHbm, int wavesPerBlock, size_t ldsBytes, int* bpc) {
+    PANN_HNSW_DISPATCH(occT, inwHbm, wavesPerBlock, ldsBytes, bpc)
 }
 
 }  // namespace pann

@@ -567,8 +567,9 @@ int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_searc
     const int maxch = G <= 64 ? 1 : G <= 128 ? 2 : 4;
     const int bits0 = hnswFilterBits(Ls);
     const bool ldsFilter = bits0 <= kFilterBitsLds;
-    const uint32_t capInw = hnswInwCapFit(Ls, maxch, ldsFilter ? bits0 : 0, kMaxLdsPerBlock);
-    const HnswLds lay = hnswLdsLayout(maxch, capInw, ldsFilter ? bits0 : 0);
+    const bool inwHbm = wr >= 2 && hnswInwCap(Ls) >= kInwHbmMinCap;
+    const uint32_t capInw = inwHbm ? hnswInwCap(Ls) : hnswInwCapFit(Ls, maxch, ldsFilter ? bits0 : 0, kMaxLdsPerBlock);
+    const HnswLds lay = hnswLdsLayout(maxch, inwHbm ? 0 : capInw, ldsFilter ? bits0 : 0);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "Ls=%d needs %u bytes of LDS per query", Ls, lay.total);
 
@@ -580,7 +581,7 @@ int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_searc
     s->wr = wr;
     s->wavesPerBlock = (int)std::min<size_t>(4, std::max<size_t>(1, kMaxLdsPerBlock / lay.total));
     int bpc = 0, cus = 0;
-    hipError_t he = beamHnswOccupancy(wr, maxch, s->wavesPerBlock, (size_t)lay.total * s->wavesPerBlock, &bpc);
+    hipError_t he = beamHnswOccupancy(wr, maxch, inwHbm, s->wavesPerBlock, (size_t)lay.total * s->wavesPerBlock, &bpc);
     r = deviceCUs(&cus);
     if (he != hipSuccess || r != PANN_OK) {
         delete s;
@@ -617,6 +618,14 @@ int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_searc
         }
     }
     H.gfilt = s->gfilt;
+    if (inwHbm) {
+        he = hipMalloc(&s->gvis, (size_t)totalWaves * capInw * 4);
+        if (he != hipSuccess) {
+            pann_searcher_free(s);
+            return setError(PANN_ERR_HIP, "allocating is_inw tables: %s", hipGetErrorString(he));
+        }
+        H.ginw = s->gvis;
+    }
     r = allocOutputs(s->out, nq, (uint32_t)std::max(1, cfg->k));
     if (r != PANN_OK) {
         pann_searcher_free(s);

@@ -115,9 +115,10 @@ int sizeLaunch(pann_batch* b, int fastKind, uint32_t ldsPerWave, int* wpbOut, in
     if ((size_t)wpb * ldsPerWave > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "%d wavefronts x %u bytes of LDS exceed 64 KB", wpb, ldsPerWave);
     int bpc = 0;
+    // HNSW: fastKind != 0 selects the kernel with is_inw in HBM
     hipError_t e = b->kind == 0
                        ? beamFlatOccupancy(b->maxch, fastKind, wpb, (size_t)ldsPerWave * wpb, &bpc)
-                       : beamHnswOccupancy(b->wr, b->maxch, wpb, (size_t)ldsPerWave * wpb, &bpc);
+                       : beamHnswOccupancy(b->wr, b->maxch, fastKind != 0, wpb, (size_t)ldsPerWave * wpb, &bpc);
     if (e != hipSuccess) return setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
     *wpbOut = wpb;
     *blocksOut = std::max(1, bpc) * cus;
@@ -271,26 +272,29 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
     b->maxch = G <= 64 ? 1 : G <= 128 ? 2 : 4;
     const int fbMax = hnswFilterBits(LsMax);
     const int fbLds = std::max(2, std::min(cfg->lds_filter_bits ? cfg->lds_filter_bits : kFilterBitsLds, fbMax));
+    // is_inw in HBM from kInwHbmMinCap slots on (then at the full derived capacity)
+    const bool inwHbm = wr >= 2 && (cfg->visited_slots ? cfg->visited_slots : hnswInwCap(LsMax)) >= kInwHbmMinCap;
     const uint32_t capInw = cfg->visited_slots ? cfg->visited_slots
-                                               : hnswInwCapFit(LsMax, b->maxch, fbLds, kMaxLdsPerBlock);
+                          : inwHbm ? hnswInwCap(LsMax) : hnswInwCapFit(LsMax, b->maxch, fbLds, kMaxLdsPerBlock);
     if (capInw < 256u * b->maxch) {
         delete b;
         return setError(PANN_ERR_CAPACITY, "visited_slots %u too small for rows of %u", capInw, G);
     }
     b->cfg.lds_filter_bits = fbLds;
     b->cfg.visited_slots = capInw;
-    const HnswLds lay = hnswLdsLayout(b->maxch, capInw, fbLds);
+    const HnswLds lay = hnswLdsLayout(b->maxch, inwHbm ? 0 : capInw, fbLds);
     if (lay.total > kMaxLdsPerBlock) {
         delete b;
         return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsMax, lay.total);
     }
     const int gbits = fbMax > fbLds ? fbMax : 0;
-    r = sizeLaunch(b, 0, lay.total, &b->wavesPerBlock, &b->blocks);
+    r = sizeLaunch(b, inwHbm ? 1 : 0, lay.total, &b->wavesPerBlock, &b->blocks);
     if (r == PANN_OK) {
         b->cfg.waves_per_block = b->wavesPerBlock;
         b->totalWaves = (uint32_t)b->blocks * b->wavesPerBlock;
         r = allocFilters(b, gbits ? ((size_t)b->totalWaves << gbits) * 4 : 0);
     }
+    if (r == PANN_OK) r = allocVisited(b, inwHbm ? (size_t)b->totalWaves * capInw * 4 : 0);
     if (r == PANN_OK) r = allocCommon(b);
     if (r != PANN_OK) {
         pann_batch_free(b);
@@ -318,6 +322,7 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
     H.ldsPerWave = lay.total;
     H.lds = lay;
     H.gfilt = b->gfilt;
+    H.ginw = inwHbm ? b->gvis : nullptr;
     H.queue = b->queue;
     H.order = b->order;
     H.totalWaves = b->totalWaves;

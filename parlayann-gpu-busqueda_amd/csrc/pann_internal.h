@@ -173,6 +173,9 @@ struct FlatLaunch {
 __host__ __device__ inline int hnswFilterBits(int ef) { return ef > 2 ? 2 * ceilLog2(ef) - 2 : 2; }
 // capacity of the exact "entered the workset" set of one layer search
 __host__ __device__ inline uint32_t hnswInwCap(int Ls) { return 1u << ceilLog2((uint64_t)16 * Ls + 256); }
+// Launches whose is_inw set has at least this many slots (ef > 112) keep it in HBM
+// (HnswLaunch::ginw): in LDS it would leave one wavefront per SIMD at ef around 256.
+constexpr uint32_t kInwHbmMinCap = 4096;
 
 struct HnswLds {
     uint32_t pend;   // pending ids (64*maxch u32)
@@ -220,6 +223,7 @@ struct HnswLaunch {
     uint32_t capInw, ldsPerWave;
     HnswLds lds;
     uint32_t* gfilt;
+    uint32_t* ginw;            // non-null: capInw is_inw slots per wave in HBM
     uint32_t* queue;
     const uint32_t* order;
     uint32_t totalWaves;
@@ -234,7 +238,8 @@ struct HnswLaunch {
 
 hipError_t launchBeamHnsw(const HnswLaunch& L, int wr, int maxch, int wavesPerBlock, int blocks,
                           hipStream_t stream);
-hipError_t beamHnswOccupancy(int wr, int maxch, int wavesPerBlock, size_t ldsBytes, int* blocksPerCU);
+hipError_t beamHnswOccupancy(int wr, int maxch, bool inwHbm, int wavesPerBlock, size_t ldsBytes,
+                             int* blocksPerCU);
 
 // Launchers implemented in the .hip translation units.
 hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int blocks,

@@ -79,6 +79,6 @@ def test_random_hnsw(gpu, seed, M0, M, entries, dup):
     base, queries = datagen.gmm_dataset(n + 200, 120, dim=64, ncomp=5, seeds=(seed, seed + 9, seed + 19))
     h = random_hnsw(n, 64, rng, M0=M0, M=M, max_level=4, num_entries=entries, dup_frac=dup,
                     relabel=True, base_n=n + 200)
-    for Ls in (8, 33, 100):
+    for Ls in (8, 33, 100, 200):
         _assert_same(_run(base, h, queries, Ls, 10), _oracle(base, h, queries, Ls, 10), 10,
                      f"seed={seed} Ls={Ls}")
