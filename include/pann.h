@@ -197,9 +197,13 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
  * it completes). */
 int pann_batch_set_queries(pann_batch* b, const int8_t* host_rows, const pann_query_params* params,
                            uint32_t nq, void* stream);
-/* One persistent launch whose resident wavefronts drain the queue. */
+/* Persistent launches whose resident wavefronts drain the queue: one per Ls class present
+ * (flat graphs: Ls <= 64, 65..128, above 128), each sized for its class, one after another
+ * on `stream`. */
 int pann_batch_launch(pann_batch* b, void* stream);
-/* Records that launch into a HIP graph (for the current nq) / replays it. */
+/* Records those launches into a HIP graph / replays it.  The graph holds the class ranges
+ * of the queries set at capture time: replay fails once set_queries changed the number of
+ * queries or their split over the classes. */
 int pann_batch_capture(pann_batch* b, void* stream);
 int pann_batch_replay(pann_batch* b, void* stream);
 int pann_batch_outputs(const pann_batch* b, pann_search_outputs* out);
