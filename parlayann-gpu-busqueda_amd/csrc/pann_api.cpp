@@ -467,14 +467,15 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
     s->cfg = *cfg;
     s->nq = nq;
     s->maxch = maxch;
-    s->wavesPerBlock = (int)std::min<size_t>(4, std::max<size_t>(1, kMaxLdsPerBlock / lay.total));
     int bpc = 0, cus = 0;
-    hipError_t he = beamFlatOccupancy(maxch, fastKind, s->wavesPerBlock, (size_t)lay.total * s->wavesPerBlock, &bpc);
-    r = deviceCUs(&cus);
-    if (he != hipSuccess || r != PANN_OK) {
+    r = pickWavesPerBlock(lay.total, [&](int wpb, size_t lds, int* b) {
+            return beamFlatOccupancy(maxch, fastKind, wpb, lds, b); }, &s->wavesPerBlock, &bpc);
+    if (r == PANN_OK) r = deviceCUs(&cus);
+    if (r != PANN_OK) {
         delete s;
-        return he != hipSuccess ? setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(he)) : r;
+        return r;
     }
+    hipError_t he = hipSuccess;
     const uint32_t want = (nq + s->wavesPerBlock - 1) / s->wavesPerBlock;
     s->blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>(want, (uint32_t)std::max(1, bpc) * cus));
     const uint32_t totalWaves = (uint32_t)s->blocks * s->wavesPerBlock;
@@ -579,14 +580,15 @@ int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_searc
     s->nq = nq;
     s->maxch = maxch;
     s->wr = wr;
-    s->wavesPerBlock = (int)std::min<size_t>(4, std::max<size_t>(1, kMaxLdsPerBlock / lay.total));
     int bpc = 0, cus = 0;
-    hipError_t he = beamHnswOccupancy(wr, maxch, inwHbm, s->wavesPerBlock, (size_t)lay.total * s->wavesPerBlock, &bpc);
-    r = deviceCUs(&cus);
-    if (he != hipSuccess || r != PANN_OK) {
+    r = pickWavesPerBlock(lay.total, [&](int wpb, size_t lds, int* b) {
+            return beamHnswOccupancy(wr, maxch, inwHbm, wpb, lds, b); }, &s->wavesPerBlock, &bpc);
+    if (r == PANN_OK) r = deviceCUs(&cus);
+    if (r != PANN_OK) {
         delete s;
-        return he != hipSuccess ? setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(he)) : r;
+        return r;
     }
+    hipError_t he = hipSuccess;
     const uint32_t want = (nq + s->wavesPerBlock - 1) / s->wavesPerBlock;
     s->blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>(want, (uint32_t)std::max(1, bpc) * cus));
     const uint32_t totalWaves = (uint32_t)s->blocks * s->wavesPerBlock;

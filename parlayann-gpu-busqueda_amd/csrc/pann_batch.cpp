@@ -1,9 +1,10 @@
 // pann_batch.cpp -- the batched persistent form: resident wavefronts drain a device queue
 // of queries that each carry their own qid, Ls, k, visit limit and cut; the launch can be
 // captured into a HIP graph and replayed; an autotuner picks the launch shape per index.
-// A flat batch whose max_Ls exceeds 64 is split by Ls class (<= 64, <= 128, the rest):
-// each class drains in a launch of its own, sized for the longest beam of that class, so
-// short beams run at the occupancy of their own LDS footprint rather than max_Ls's.
+// A batch is split by Ls class -- flat: <= 64, <= 128, the rest; HNSW: the widths of its
+// register-resident lists, Ls <= 55, <= 119, <= 247, the rest -- and each class drains in a
+// launch of its own, sized for the longest beam of that class, so short beams run at the
+// occupancy and list width of their own class rather than max_Ls's.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -13,14 +14,16 @@
 
 using namespace pann;
 
-// One launch of a flat batch: the queries with LsLo < Ls <= LsHi, a contiguous range of the
+// One launch of a batch: the queries with LsLo < Ls <= LsHi, a contiguous range of the
 // Ls-descending order, drained by a kernel sized for LsHi.
-struct FlatPart {
+struct BatchPart {
     int LsLo = 0, LsHi = 0;
-    int fastKind = 0;  // FlatLaunch::fast of the register-resident kernel (0: general)
+    int fastKind = 0;  // flat: FlatLaunch::fast (0: general); HNSW: 1 with is_inw in HBM
+    int wr = 1;        // HNSW: list width in wavefront registers
     int wavesPerBlock = 1, blocks = 0;
     uint32_t begin = 0, count = 0;
     FlatLaunch L{};
+    HnswLaunch H{};
 };
 
 struct pann_batch {
@@ -32,14 +35,13 @@ struct pann_batch {
     uint32_t maxQueries = 0, nq = 0, dim = 0;
     int maxch = 1, wr = 1, wavesPerBlock = 1, blocks = 0;
     uint32_t totalWaves = 0;
-    std::vector<FlatPart> parts;  // flat: one per Ls class, ascending LsHi
-    HnswLaunch H{};
+    std::vector<BatchPart> parts;  // one per Ls class, ascending LsHi
     int8_t* queries = nullptr;
     QueryParams* params = nullptr;
     uint32_t* order = nullptr;
     uint32_t* queue = nullptr;
     uint32_t* gfilt = nullptr;
-    uint32_t* gvis = nullptr;  // HBM visited tables of the FlatLaunch::fast -1 launches
+    uint32_t* gvis = nullptr;  // HBM visited / is_inw tables of the launches that use them
     pann_search_outputs out{};
     std::vector<uint32_t> hostOrder;
     hipGraph_t graphDef = nullptr;
@@ -91,7 +93,7 @@ int allocCommon(pann_batch* b) {
 
 std::vector<uint32_t> partCounts(const pann_batch* b) {
     std::vector<uint32_t> c;
-    for (const FlatPart& P : b->parts) c.push_back(P.count);
+    for (const BatchPart& P : b->parts) c.push_back(P.count);
     return c;
 }
 
@@ -106,20 +108,26 @@ int allocVisited(pann_batch* b, size_t bytes) {
 }
 
 // Wavefronts per block and resident blocks for one kernel of the batch.
-int sizeLaunch(pann_batch* b, int fastKind, uint32_t ldsPerWave, int* wpbOut, int* blocksOut) {
+int sizeLaunch(pann_batch* b, int fastKind, int wr, uint32_t ldsPerWave, int* wpbOut, int* blocksOut) {
     int cus = 0;
     int r = deviceCUs(&cus);
     if (r != PANN_OK) return r;
-    int wpb = b->cfg.waves_per_block;
-    if (wpb == 0) wpb = (int)std::min<size_t>(4, std::max<size_t>(1, kMaxLdsPerBlock / ldsPerWave));
-    if ((size_t)wpb * ldsPerWave > kMaxLdsPerBlock)
-        return setError(PANN_ERR_CAPACITY, "%d wavefronts x %u bytes of LDS exceed 64 KB", wpb, ldsPerWave);
-    int bpc = 0;
     // HNSW: fastKind != 0 selects the kernel with is_inw in HBM
-    hipError_t e = b->kind == 0
-                       ? beamFlatOccupancy(b->maxch, fastKind, wpb, (size_t)ldsPerWave * wpb, &bpc)
-                       : beamHnswOccupancy(b->wr, b->maxch, fastKind != 0, wpb, (size_t)ldsPerWave * wpb, &bpc);
-    if (e != hipSuccess) return setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
+    auto occ = [&](int w, size_t lds, int* bpc) {
+        return b->kind == 0 ? beamFlatOccupancy(b->maxch, fastKind, w, lds, bpc)
+                            : beamHnswOccupancy(wr, b->maxch, fastKind != 0, w, lds, bpc);
+    };
+    int wpb = b->cfg.waves_per_block;
+    int bpc = 0;
+    if (wpb == 0) {
+        r = pickWavesPerBlock(ldsPerWave, occ, &wpb, &bpc);
+        if (r != PANN_OK) return r;
+    } else {
+        if ((size_t)wpb * ldsPerWave > kMaxLdsPerBlock)
+            return setError(PANN_ERR_CAPACITY, "%d wavefronts x %u bytes of LDS exceed 64 KB", wpb, ldsPerWave);
+        const hipError_t e = occ(wpb, (size_t)ldsPerWave * wpb, &bpc);
+        if (e != hipSuccess) return setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
+    }
     *wpbOut = wpb;
     *blocksOut = std::max(1, bpc) * cus;
     return PANN_OK;
@@ -128,7 +136,7 @@ int sizeLaunch(pann_batch* b, int fastKind, uint32_t ldsPerWave, int* wpbOut, in
 // Sizes one flat launch for beams up to LsHi under the batch's configuration.  The visited
 // hash and the LDS filter never exceed what LsHi itself derives.  Records the launch's
 // global filter bits (0: none) in *gbits.
-int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, FlatPart* P, int* gbits) {
+int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int* gbits) {
     const pann_batch_config& c = b->cfg;
     const int fbMax = flatFilterBits(LsHi);
     const int fbLds = std::min(c.lds_filter_bits ? c.lds_filter_bits : kFilterBitsLds, fbMax);
@@ -149,7 +157,7 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, FlatPart* P, int* 
                               : flatLdsLayout(LsHi, b->maxch, hbmVisited ? 0 : capV, fbLaunch);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsHi, lay.total);
-    int r = sizeLaunch(b, P->fastKind, lay.total, &P->wavesPerBlock, &P->blocks);
+    int r = sizeLaunch(b, P->fastKind, 1, lay.total, &P->wavesPerBlock, &P->blocks);
     if (r != PANN_OK) return r;
     *gbits = (fbMax > fbLds || fast2) ? fbMax : 0;
     FlatLaunch& L = P->L;
@@ -167,6 +175,83 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, FlatPart* P, int* 
         b->cfg.visited_slots = capV;
         if (!fast) b->cfg.prefetch = 0;
     }
+    return PANN_OK;
+}
+
+// List width (wavefront registers) of the HNSW kernel for beams up to Ls (0: too wide).
+int hnswListWidth(int Ls, uint32_t numEntries) {
+    const int need = std::max<int>(Ls, (int)numEntries) + 9;
+    return need <= 64 ? 1 : need <= 128 ? 2 : need <= 256 ? 4 : need <= 512 ? 8 : 0;
+}
+
+// Sizes one HNSW launch for beams up to LsHi (as planFlatPart).  Records the launch's
+// global filter bits in *gbits and its HBM is_inw slots per wave (0: LDS) in *ginwSlots.
+int planHnswPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int* gbits, uint32_t* ginwSlots) {
+    const pann_batch_config& c = b->cfg;
+    const pann_hnsw* h = b->hnsw;
+    const int wr = hnswListWidth(LsHi, h->numEntries);
+    if (!wr) return setError(PANN_ERR_CAPACITY, "HNSW max_Ls=%d above the supported 503", LsHi);
+    const int fbMax = hnswFilterBits(LsHi);
+    const int fbLds = std::max(2, std::min(c.lds_filter_bits ? c.lds_filter_bits : kFilterBitsLds, fbMax));
+    // is_inw in HBM from kInwHbmMinCap slots on (then at the full derived capacity)
+    const uint32_t derived = hnswInwCap(LsHi);
+    const uint32_t want = !c.visited_slots ? derived : top ? c.visited_slots : std::min(c.visited_slots, derived);
+    const bool inwHbm = wr >= 2 && want >= kInwHbmMinCap;
+    const uint32_t capInw = c.visited_slots || inwHbm ? want : hnswInwCapFit(LsHi, b->maxch, fbLds, kMaxLdsPerBlock);
+    if (capInw < 256u * b->maxch)
+        return setError(PANN_ERR_CAPACITY, "visited_slots %u too small for rows of %u", capInw,
+                        std::max(h->G0, h->GA));
+    // a launch whose shortest beam already needs a larger layer-0 filter keeps only the
+    // 4-slot upper-layer filter in LDS
+    const int fbLaunch = hnswFilterBits(LsLo + 1) > fbLds ? 2 : fbLds;
+    const HnswLds lay = hnswLdsLayout(b->maxch, inwHbm ? 0 : capInw, fbLaunch);
+    if (lay.total > kMaxLdsPerBlock)
+        return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsHi, lay.total);
+    P->LsLo = LsLo;
+    P->LsHi = LsHi;
+    P->wr = wr;
+    P->fastKind = inwHbm ? 1 : 0;
+    int r = sizeLaunch(b, P->fastKind, wr, lay.total, &P->wavesPerBlock, &P->blocks);
+    if (r != PANN_OK) return r;
+    *gbits = fbMax > fbLds ? fbMax : 0;
+    *ginwSlots = inwHbm ? capInw : 0;
+    HnswLaunch& H = P->H;
+    H.vec = h->nodeVec;
+    H.normsInRow = h->dim <= (uint32_t)kNormInRowMaxDim;
+    H.adj0 = h->adj0;
+    H.adjUp = h->adjUp;
+    H.upStart = h->upStart;
+    H.nodePoint = h->nodePoint;
+    H.n = h->n;
+    H.G0 = h->G0;
+    H.GA = h->GA;
+    H.top = (int32_t)h->top;
+    H.numEntries = (int32_t)h->numEntries;
+    for (int e = 0; e < 8; e++) H.entries[e] = e < (int)h->numEntries ? h->entries[e] : 0;
+    H.LsMax = LsHi;
+    H.fbitsLds = fbLaunch;
+    H.gbitsMax = *gbits;
+    H.capInw = capInw;
+    H.ldsPerWave = lay.total;
+    H.lds = lay;
+    if (top) {
+        b->cfg.lds_filter_bits = fbLds;
+        b->cfg.visited_slots = capInw;
+    }
+    return PANN_OK;
+}
+
+// Allocates the shared scratch of the parts and the common buffers, then points every
+// part's launch at them.
+int finishParts(pann_batch* b, size_t gfiltBytes, size_t gvisBytes) {
+    int r = allocFilters(b, gfiltBytes);
+    if (r == PANN_OK) r = allocVisited(b, gvisBytes);
+    if (r == PANN_OK) r = allocCommon(b);
+    if (r != PANN_OK) return r;
+    const BatchPart& top = b->parts.back();
+    b->wavesPerBlock = top.wavesPerBlock;
+    b->blocks = top.blocks;
+    b->cfg.waves_per_block = top.wavesPerBlock;
     return PANN_OK;
 }
 
@@ -205,25 +290,19 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
             pann_batch_free(b);
             return r;
         }
-        const FlatPart& P = b->parts[i];
+        const BatchPart& P = b->parts[i];
         const uint32_t waves = (uint32_t)P.blocks * P.wavesPerBlock;
         b->totalWaves = std::max(b->totalWaves, waves);
         if (gbits) gfiltBytes = std::max(gfiltBytes, ((size_t)waves << gbits) * 4);
         if (P.fastKind < 0) gvisBytes = std::max(gvisBytes, (size_t)waves * P.L.capV * 5);
     }
-    const FlatPart& top = b->parts.back();
-    b->wavesPerBlock = top.wavesPerBlock;
-    b->blocks = top.blocks;
-    b->cfg.waves_per_block = top.wavesPerBlock;
-    // the launches run one after another on one stream and share the global filters
-    r = allocFilters(b, gfiltBytes);
-    if (r == PANN_OK) r = allocVisited(b, gvisBytes);
-    if (r == PANN_OK) r = allocCommon(b);
+    // the launches run one after another on one stream and share the HBM scratch
+    r = finishParts(b, gfiltBytes, gvisBytes);
     if (r != PANN_OK) {
         pann_batch_free(b);
         return r;
     }
-    for (FlatPart& P : b->parts) {
+    for (BatchPart& P : b->parts) {
         FlatLaunch& L = P.L;
         L.gfilt = b->gfilt;
         if (P.fastKind < 0) {
@@ -257,9 +336,8 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
     int r = checkBatchConfig(cfg);
     if (r != PANN_OK) return r;
     const int LsMax = cfg->max_Ls;
-    const int need = std::max<int>(LsMax, (int)h->numEntries) + 9;
-    const int wr = need <= 64 ? 1 : need <= 128 ? 2 : need <= 256 ? 4 : need <= 512 ? 8 : 0;
-    if (!wr) return setError(PANN_ERR_CAPACITY, "HNSW max_Ls=%d above the supported 503", LsMax);
+    if (!hnswListWidth(LsMax, h->numEntries))
+        return setError(PANN_ERR_CAPACITY, "HNSW max_Ls=%d above the supported 503", LsMax);
     pann_batch* b = new pann_batch();
     b->kind = 1;
     b->hnsw = h;
@@ -267,72 +345,49 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
     b->cfg.prefetch = 0;  // only the flat register-resident form prefetches
     b->maxQueries = max_queries;
     b->dim = h->dim;
-    b->wr = wr;
     const uint32_t G = std::max(h->G0, h->GA);
     b->maxch = G <= 64 ? 1 : G <= 128 ? 2 : 4;
-    const int fbMax = hnswFilterBits(LsMax);
-    const int fbLds = std::max(2, std::min(cfg->lds_filter_bits ? cfg->lds_filter_bits : kFilterBitsLds, fbMax));
-    // is_inw in HBM from kInwHbmMinCap slots on (then at the full derived capacity)
-    const bool inwHbm = wr >= 2 && (cfg->visited_slots ? cfg->visited_slots : hnswInwCap(LsMax)) >= kInwHbmMinCap;
-    const uint32_t capInw = cfg->visited_slots ? cfg->visited_slots
-                          : inwHbm ? hnswInwCap(LsMax) : hnswInwCapFit(LsMax, b->maxch, fbLds, kMaxLdsPerBlock);
-    if (capInw < 256u * b->maxch) {
-        delete b;
-        return setError(PANN_ERR_CAPACITY, "visited_slots %u too small for rows of %u", capInw, G);
+    // classes by list width: Ls + 9 within 64, 128, 256 entries
+    std::vector<int> his;
+    for (int hi : {55, 119, 247})
+        if (hi < LsMax) his.push_back(hi);
+    his.push_back(LsMax);
+    b->parts.resize(his.size());
+    size_t gfiltBytes = 0, gvisBytes = 0;
+    for (size_t i = 0; i < his.size(); i++) {
+        int gbits = 0;
+        uint32_t inwSlots = 0;
+        r = planHnswPart(b, i ? his[i - 1] : 0, his[i], i + 1 == his.size(), &b->parts[i], &gbits, &inwSlots);
+        if (r != PANN_OK) {
+            pann_batch_free(b);
+            return r;
+        }
+        const BatchPart& P = b->parts[i];
+        const uint32_t waves = (uint32_t)P.blocks * P.wavesPerBlock;
+        b->totalWaves = std::max(b->totalWaves, waves);
+        if (gbits) gfiltBytes = std::max(gfiltBytes, ((size_t)waves << gbits) * 4);
+        gvisBytes = std::max(gvisBytes, (size_t)waves * inwSlots * 4);
     }
-    b->cfg.lds_filter_bits = fbLds;
-    b->cfg.visited_slots = capInw;
-    const HnswLds lay = hnswLdsLayout(b->maxch, inwHbm ? 0 : capInw, fbLds);
-    if (lay.total > kMaxLdsPerBlock) {
-        delete b;
-        return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsMax, lay.total);
-    }
-    const int gbits = fbMax > fbLds ? fbMax : 0;
-    r = sizeLaunch(b, inwHbm ? 1 : 0, lay.total, &b->wavesPerBlock, &b->blocks);
-    if (r == PANN_OK) {
-        b->cfg.waves_per_block = b->wavesPerBlock;
-        b->totalWaves = (uint32_t)b->blocks * b->wavesPerBlock;
-        r = allocFilters(b, gbits ? ((size_t)b->totalWaves << gbits) * 4 : 0);
-    }
-    if (r == PANN_OK) r = allocVisited(b, inwHbm ? (size_t)b->totalWaves * capInw * 4 : 0);
-    if (r == PANN_OK) r = allocCommon(b);
+    r = finishParts(b, gfiltBytes, gvisBytes);
     if (r != PANN_OK) {
         pann_batch_free(b);
         return r;
     }
-    HnswLaunch& H = b->H;
-    H.vec = h->nodeVec;
-    H.normsInRow = h->dim <= (uint32_t)kNormInRowMaxDim;
-    H.adj0 = h->adj0;
-    H.adjUp = h->adjUp;
-    H.upStart = h->upStart;
-    H.nodePoint = h->nodePoint;
-    H.n = h->n;
-    H.G0 = h->G0;
-    H.GA = h->GA;
-    H.top = (int32_t)h->top;
-    H.numEntries = (int32_t)h->numEntries;
-    for (int e = 0; e < 8; e++) H.entries[e] = e < (int)h->numEntries ? h->entries[e] : 0;
-    H.queries = b->queries;
-    H.qp = b->params;
-    H.LsMax = LsMax;
-    H.fbitsLds = fbLds;
-    H.gbitsMax = gbits;
-    H.capInw = capInw;
-    H.ldsPerWave = lay.total;
-    H.lds = lay;
-    H.gfilt = b->gfilt;
-    H.ginw = inwHbm ? b->gvis : nullptr;
-    H.queue = b->queue;
-    H.order = b->order;
-    H.totalWaves = b->totalWaves;
-    H.outIds = b->out.ids;
-    H.outDists = b->out.dists;
-    H.outStride = b->out.stride;
-    H.outComps = b->out.comps;
-    H.outFlags = b->out.flags;
-    H.outEdges = b->out.edges;
-    H.outHops = b->out.hops;
+    b->wr = b->parts.back().wr;
+    for (BatchPart& P : b->parts) {
+        HnswLaunch& H = P.H;
+        H.queries = b->queries;
+        H.qp = b->params;
+        H.gfilt = b->gfilt;
+        H.ginw = P.fastKind ? b->gvis : nullptr;
+        H.outIds = b->out.ids;
+        H.outDists = b->out.dists;
+        H.outStride = b->out.stride;
+        H.outComps = b->out.comps;
+        H.outFlags = b->out.flags;
+        H.outEdges = b->out.edges;
+        H.outHops = b->out.hops;
+    }
     *out = b;
     return PANN_OK;
 }
@@ -357,7 +412,7 @@ int pann_batch_set_queries(pann_batch* b, const int8_t* host_rows, const pann_qu
     std::stable_sort(b->hostOrder.begin(), b->hostOrder.end(),
                      [&](uint32_t x, uint32_t y) { return params[x].Ls > params[y].Ls; });
     // each flat launch takes the range of that order whose Ls falls in its class
-    for (FlatPart& P : b->parts) {
+    for (BatchPart& P : b->parts) {
         P.begin = P.count = 0;
         for (uint32_t i = 0; i < nq; i++) {
             const int Ls = params[i].Ls;
@@ -376,27 +431,28 @@ int pann_batch_launch(pann_batch* b, void* stream) {
     if (!b) return setError(PANN_ERR_ARG, "batch is NULL");
     if (b->nq == 0) return PANN_OK;
     hipError_t e = hipSuccess;
-    if (b->kind == 0) {
-        // longest class first; a small class launches only as many wavefronts as it has queries
-        for (size_t i = b->parts.size(); i-- > 0 && e == hipSuccess;) {
-            const FlatPart& P = b->parts[i];
-            if (P.count == 0) continue;
-            const int blocks = (int)std::min<uint32_t>((uint32_t)P.blocks,
-                                                       (P.count + P.wavesPerBlock - 1) / P.wavesPerBlock);
+    // longest class first; a small class launches only as many wavefronts as it has queries
+    for (size_t i = b->parts.size(); i-- > 0 && e == hipSuccess;) {
+        const BatchPart& P = b->parts[i];
+        if (P.count == 0) continue;
+        const int blocks = (int)std::min<uint32_t>((uint32_t)P.blocks,
+                                                   (P.count + P.wavesPerBlock - 1) / P.wavesPerBlock);
+        const uint32_t waves = (uint32_t)blocks * P.wavesPerBlock;
+        if (b->kind == 0) {
             FlatLaunch L = P.L;
             L.nq = P.count;
             L.order = b->order + P.begin;
             L.queue = b->queue + 2 * i;
-            L.totalWaves = (uint32_t)blocks * P.wavesPerBlock;
+            L.totalWaves = waves;
             e = launchBeamFlat(L, b->maxch, P.wavesPerBlock, blocks, (hipStream_t)stream);
+        } else {
+            HnswLaunch H = P.H;
+            H.nq = P.count;
+            H.order = b->order + P.begin;
+            H.queue = b->queue + 2 * i;
+            H.totalWaves = waves;
+            e = launchBeamHnsw(H, P.wr, b->maxch, P.wavesPerBlock, blocks, (hipStream_t)stream);
         }
-    } else {
-        const int blocks = (int)std::min<uint32_t>((uint32_t)b->blocks,
-                                                   (b->nq + b->wavesPerBlock - 1) / b->wavesPerBlock);
-        HnswLaunch H = b->H;
-        H.nq = b->nq;
-        H.totalWaves = (uint32_t)blocks * b->wavesPerBlock;
-        e = launchBeamHnsw(H, b->wr, b->maxch, b->wavesPerBlock, blocks, (hipStream_t)stream);
     }
     if (e != hipSuccess) return setError(PANN_ERR_HIP, "batched launch: %s", hipGetErrorString(e));
     return PANN_OK;

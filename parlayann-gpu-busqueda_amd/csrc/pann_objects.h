@@ -71,4 +71,27 @@ void freeOutputs(pann_search_outputs& o);
 constexpr int kFilterBitsLds = 10;
 constexpr size_t kMaxLdsPerBlock = 64 * 1024;
 
+// Wavefronts per block (4, 2 or 1, within 64 KB of LDS) that leaves the most wavefronts
+// resident on a CU, ties to the larger block.  A CU holds 160 KB of LDS, so the largest
+// block is not always best: at 13.5 KB per wavefront, two 4-wave blocks fit (8 wavefronts)
+// but eleven 1-wave blocks do.  occ(wpb, ldsBytes, &blocksPerCU) is the kernel's
+// occupancy query.
+template <class Occ>
+int pickWavesPerBlock(uint32_t ldsPerWave, Occ occ, int* wpbOut, int* blocksPerCU) {
+    int best = -1;
+    for (int wpb : {4, 2, 1}) {
+        if ((size_t)wpb * ldsPerWave > kMaxLdsPerBlock) continue;
+        int bpc = 0;
+        const hipError_t e = occ(wpb, (size_t)ldsPerWave * wpb, &bpc);
+        if (e != hipSuccess) return setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
+        if (bpc * wpb > best) {
+            best = bpc * wpb;
+            *wpbOut = wpb;
+            *blocksPerCU = bpc;
+        }
+    }
+    if (best < 0) return setError(PANN_ERR_CAPACITY, "%u bytes of LDS per wavefront exceed 64 KB", ldsPerWave);
+    return PANN_OK;
+}
+
 }  // namespace pann

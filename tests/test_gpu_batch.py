@@ -69,16 +69,19 @@ def test_hnsw_batch_matches_oracle(gpu):
     base, queries = datagen.gmm_dataset(4000, 300, dim=100, ncomp=12, seeds=(71, 72, 73))
     h = builders.build_hnsw(base, M=16, efc=64, seed=2)
     rng = np.random.default_rng(2)
-    params = _mixed_params(len(queries), len(base), rng, ls_hi=300)
     b = search.Vectors(base)
     idx = search.HnswIndex(h, b)
     batch = search.Batch(len(queries), 300, 100, hnsw=idx)
-    batch.set_queries(queries, params)
-    batch.launch()
-    search.synchronize()
-    r = batch.download()
-    o = O.search_hnsw(base, h, queries, _oracle_params(params))
-    _check_against_oracle(r, o, params, "hnsw batch")
+    # all four list-width classes (Ls <= 55, <= 119, <= 247, above), then only the widest
+    # and only the narrowest
+    for lo, hi in [(10, 300), (248, 300), (1, 55)]:
+        params = _mixed_params(len(queries), len(base), rng, ls_lo=lo, ls_hi=hi)
+        batch.set_queries(queries, params)
+        batch.launch()
+        search.synchronize()
+        r = batch.download()
+        o = O.search_hnsw(base, h, queries, _oracle_params(params))
+        _check_against_oracle(r, o, params, f"hnsw batch Ls {lo}..{hi}")
     batch.free(), idx.free(), b.free()
 
 
