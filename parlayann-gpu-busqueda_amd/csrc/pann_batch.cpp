@@ -127,6 +127,7 @@ int sizeLaunch(pann_batch* b, int fastKind, int wr, uint32_t ldsPerWave, int* wp
             return setError(PANN_ERR_CAPACITY, "%d wavefronts x %u bytes of LDS exceed 64 KB", wpb, ldsPerWave);
         const hipError_t e = occ(wpb, (size_t)ldsPerWave * wpb, &bpc);
         if (e != hipSuccess) return setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
+        bpc = std::min(bpc, ldsBlocksPerCU((size_t)ldsPerWave * wpb));
     }
     *wpbOut = wpb;
     *blocksOut = std::max(1, bpc) * cus;

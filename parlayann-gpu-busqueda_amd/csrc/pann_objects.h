@@ -2,6 +2,7 @@
 // translation units of libpann.so.
 #pragma once
 
+#include <algorithm>
 #include <string>
 
 #include "../../include/pann.h"
@@ -71,11 +72,19 @@ void freeOutputs(pann_search_outputs& o);
 constexpr int kFilterBitsLds = 10;
 constexpr size_t kMaxLdsPerBlock = 64 * 1024;
 
+// Blocks of `bytes` of LDS that are resident on one CU at once.  A gfx950 CU holds 160 KB
+// of LDS, handed to workgroups in 2 KB granules: three 54,272-B blocks do not fit (a launch
+// of three per CU takes two rounds) while three 53,248-B and four 40,960-B blocks do.  The
+// runtime's occupancy query counts bytes, so its answer is capped by this.
+inline int ldsBlocksPerCU(size_t bytes) {
+    if (bytes == 0) return 1 << 30;
+    const size_t granted = (bytes + 2047) & ~(size_t)2047;
+    return (int)((size_t)160 * 1024 / granted);
+}
+
 // Wavefronts per block (4, 2 or 1, within 64 KB of LDS) that leaves the most wavefronts
-// resident on a CU, ties to the larger block.  A CU holds 160 KB of LDS, so the largest
-// block is not always best: at 13.5 KB per wavefront, two 4-wave blocks fit (8 wavefronts)
-// but eleven 1-wave blocks do.  occ(wpb, ldsBytes, &blocksPerCU) is the kernel's
-// occupancy query.
+// resident on a CU, ties to the larger block; *blocksPerCU is the resident block count.
+// occ(wpb, ldsBytes, &blocksPerCU) is the kernel's occupancy query.
 template <class Occ>
 int pickWavesPerBlock(uint32_t ldsPerWave, Occ occ, int* wpbOut, int* blocksPerCU) {
     int best = -1;
@@ -84,6 +93,7 @@ int pickWavesPerBlock(uint32_t ldsPerWave, Occ occ, int* wpbOut, int* blocksPerC
         int bpc = 0;
         const hipError_t e = occ(wpb, (size_t)ldsPerWave * wpb, &bpc);
         if (e != hipSuccess) return setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
+        bpc = std::min(bpc, ldsBlocksPerCU((size_t)ldsPerWave * wpb));
         if (bpc * wpb > best) {
             best = bpc * wpb;
             *wpbOut = wpb;
