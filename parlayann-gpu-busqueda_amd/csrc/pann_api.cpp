@@ -561,8 +561,7 @@ int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_searc
     int r = checkConfig(cfg);
     if (r != PANN_OK) return r;
     const int Ls = cfg->Ls;
-    const int need = std::max<int>(Ls, (int)h->numEntries) + 9;
-    const int wr = need <= 64 ? 1 : need <= 128 ? 2 : need <= 256 ? 4 : need <= 512 ? 8 : 0;
+    const int wr = hnswListWidth(Ls, h->numEntries);
     if (!wr) return setError(PANN_ERR_CAPACITY, "HNSW Ls=%d above the supported 503", Ls);
     const uint32_t G = std::max(h->G0, h->GA);
     const int maxch = G <= 64 ? 1 : G <= 128 ? 2 : 4;

@@ -2,7 +2,7 @@
 // of queries that each carry their own qid, Ls, k, visit limit and cut; the launch can be
 // captured into a HIP graph and replayed; an autotuner picks the launch shape per index.
 // A batch is split by Ls class -- flat: <= 64, <= 128, the rest; HNSW: the widths of its
-// register-resident lists, Ls <= 55, <= 119, <= 247, the rest -- and each class drains in a
+// register-resident lists, Ls <= 55, <= 119, <= 247, <= 311, the rest -- and each class drains in a
 // launch of its own, sized for the longest beam of that class, so short beams run at the
 // occupancy and list width of their own class rather than max_Ls's.
 #include <algorithm>
@@ -179,12 +179,6 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int*
     return PANN_OK;
 }
 
-// List width (wavefront registers) of the HNSW kernel for beams up to Ls (0: too wide).
-int hnswListWidth(int Ls, uint32_t numEntries) {
-    const int need = std::max<int>(Ls, (int)numEntries) + 9;
-    return need <= 64 ? 1 : need <= 128 ? 2 : need <= 256 ? 4 : need <= 512 ? 8 : 0;
-}
-
 // Sizes one HNSW launch for beams up to LsHi (as planFlatPart).  Records the launch's
 // global filter bits in *gbits and its HBM is_inw slots per wave (0: LDS) in *ginwSlots.
 int planHnswPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int* gbits, uint32_t* ginwSlots) {
@@ -348,9 +342,9 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
     b->dim = h->dim;
     const uint32_t G = std::max(h->G0, h->GA);
     b->maxch = G <= 64 ? 1 : G <= 128 ? 2 : 4;
-    // classes by list width: Ls + 9 within 64, 128, 256 entries
+    // classes by list width: Ls + 9 within 64, 128, 256, 320 entries
     std::vector<int> his;
-    for (int hi : {55, 119, 247})
+    for (int hi : {55, 119, 247, 311})
         if (hi < LsMax) his.push_back(hi);
     his.push_back(LsMax);
     b->parts.resize(his.size());

@@ -173,6 +173,13 @@ struct FlatLaunch {
 __host__ __device__ inline int hnswFilterBits(int ef) { return ef > 2 ? 2 * ceilLog2(ef) - 2 : 2; }
 // capacity of the exact "entered the workset" set of one layer search
 __host__ __device__ inline uint32_t hnswInwCap(int Ls) { return 1u << ceilLog2((uint64_t)16 * Ls + 256); }
+// Width in wavefront registers (64 entries each) of the HNSW kernel's W and cand lists for
+// beams up to Ls: Ls + 9 entries (the entry points and a margin) must fit; 0 = too wide.
+inline int hnswListWidth(int Ls, uint32_t numEntries) {
+    const int need = (Ls > (int)numEntries ? Ls : (int)numEntries) + 9;
+    return need <= 64 ? 1 : need <= 128 ? 2 : need <= 256 ? 4 : need <= 320 ? 5 : need <= 512 ? 8 : 0;
+}
+
 // Launches whose is_inw set has at least this many slots (ef > 112) keep it in HBM
 // (HnswLaunch::ginw): in LDS it would leave one wavefront per SIMD at ef around 256.
 constexpr uint32_t kInwHbmMinCap = 4096;
