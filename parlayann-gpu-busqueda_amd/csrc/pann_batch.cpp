@@ -4,7 +4,9 @@
 // A batch is split by Ls class -- flat: <= 64, <= 128, the rest; HNSW: the widths of its
 // register-resident lists, Ls <= 55, <= 119, <= 247, <= 311, the rest -- and each class drains in a
 // launch of its own, sized for the longest beam of that class, so short beams run at the
-// occupancy and list width of their own class rather than max_Ls's.
+// occupancy and list width of their own class rather than max_Ls's.  The class launches
+// run side by side (the narrower classes on the batch's own streams, forked from and
+// joined back to the caller's), each with its own HBM scratch.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -22,6 +24,7 @@ struct BatchPart {
     int wr = 1;        // HNSW: list width in wavefront registers
     int wavesPerBlock = 1, blocks = 0;
     uint32_t begin = 0, count = 0;
+    size_t gfiltOff = 0, gvisOff = 0;  // this launch's HBM scratch within the batch's regions
     FlatLaunch L{};
     HnswLaunch H{};
 };
@@ -48,6 +51,9 @@ struct pann_batch {
     hipGraphExec_t graphExec = nullptr;
     uint32_t capturedNq = 0;
     std::vector<uint32_t> capturedCounts;
+    std::vector<hipStream_t> side;  // parts[i] (but the widest) launch on side[i]
+    hipEvent_t forkEv = nullptr;
+    std::vector<hipEvent_t> joinEv;
 };
 
 namespace {
@@ -243,6 +249,16 @@ int finishParts(pann_batch* b, size_t gfiltBytes, size_t gvisBytes) {
     if (r == PANN_OK) r = allocVisited(b, gvisBytes);
     if (r == PANN_OK) r = allocCommon(b);
     if (r != PANN_OK) return r;
+    if (b->parts.size() > 1) {
+        // default-priority streams: the class launches are independent and run side by side
+        b->side.assign(b->parts.size() - 1, nullptr);
+        b->joinEv.assign(b->parts.size() - 1, nullptr);
+        for (size_t i = 0; i + 1 < b->parts.size(); i++) {
+            HIP_TRY(hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&b->joinEv[i], hipEventDisableTiming));
+        }
+        HIP_TRY(hipEventCreateWithFlags(&b->forkEv, hipEventDisableTiming));
+    }
     const BatchPart& top = b->parts.back();
     b->wavesPerBlock = top.wavesPerBlock;
     b->blocks = top.blocks;
@@ -285,13 +301,14 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
             pann_batch_free(b);
             return r;
         }
-        const BatchPart& P = b->parts[i];
+        BatchPart& P = b->parts[i];
         const uint32_t waves = (uint32_t)P.blocks * P.wavesPerBlock;
         b->totalWaves = std::max(b->totalWaves, waves);
-        if (gbits) gfiltBytes = std::max(gfiltBytes, ((size_t)waves << gbits) * 4);
-        if (P.fastKind < 0) gvisBytes = std::max(gvisBytes, (size_t)waves * P.L.capV * 5);
+        P.gfiltOff = gfiltBytes;
+        P.gvisOff = gvisBytes;
+        if (gbits) gfiltBytes += ((size_t)waves << gbits) * 4;
+        if (P.fastKind < 0) gvisBytes += (size_t)waves * P.L.capV * 5;
     }
-    // the launches run one after another on one stream and share the HBM scratch
     r = finishParts(b, gfiltBytes, gvisBytes);
     if (r != PANN_OK) {
         pann_batch_free(b);
@@ -299,11 +316,11 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
     }
     for (BatchPart& P : b->parts) {
         FlatLaunch& L = P.L;
-        L.gfilt = b->gfilt;
+        L.gfilt = L.gbitsMax ? reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(b->gfilt) + P.gfiltOff) : nullptr;
         if (P.fastKind < 0) {
             const size_t slots = (size_t)P.blocks * P.wavesPerBlock * L.capV;
-            L.gvkey = b->gvis;
-            L.gvcnt = reinterpret_cast<unsigned char*>(b->gvis + slots);
+            L.gvkey = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(b->gvis) + P.gvisOff);
+            L.gvcnt = reinterpret_cast<unsigned char*>(L.gvkey + slots);
         }
         L.qp = b->params;
         L.qids = nullptr;
@@ -357,11 +374,13 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
             pann_batch_free(b);
             return r;
         }
-        const BatchPart& P = b->parts[i];
+        BatchPart& P = b->parts[i];
         const uint32_t waves = (uint32_t)P.blocks * P.wavesPerBlock;
         b->totalWaves = std::max(b->totalWaves, waves);
-        if (gbits) gfiltBytes = std::max(gfiltBytes, ((size_t)waves << gbits) * 4);
-        gvisBytes = std::max(gvisBytes, (size_t)waves * inwSlots * 4);
+        P.gfiltOff = gfiltBytes;
+        P.gvisOff = gvisBytes;
+        if (gbits) gfiltBytes += ((size_t)waves << gbits) * 4;
+        gvisBytes += (size_t)waves * inwSlots * 4;
     }
     r = finishParts(b, gfiltBytes, gvisBytes);
     if (r != PANN_OK) {
@@ -373,8 +392,8 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
         HnswLaunch& H = P.H;
         H.queries = b->queries;
         H.qp = b->params;
-        H.gfilt = b->gfilt;
-        H.ginw = P.fastKind ? b->gvis : nullptr;
+        H.gfilt = H.gbitsMax ? reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(b->gfilt) + P.gfiltOff) : nullptr;
+        H.ginw = P.fastKind ? reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(b->gvis) + P.gvisOff) : nullptr;
         H.outIds = b->out.ids;
         H.outDists = b->out.dists;
         H.outStride = b->out.stride;
@@ -406,13 +425,22 @@ int pann_batch_set_queries(pann_batch* b, const int8_t* host_rows, const pann_qu
     std::iota(b->hostOrder.begin(), b->hostOrder.end(), 0u);
     std::stable_sort(b->hostOrder.begin(), b->hostOrder.end(),
                      [&](uint32_t x, uint32_t y) { return params[x].Ls > params[y].Ls; });
-    // each flat launch takes the range of that order whose Ls falls in its class
-    for (BatchPart& P : b->parts) {
-        P.begin = P.count = 0;
-        for (uint32_t i = 0; i < nq; i++) {
-            const int Ls = params[i].Ls;
-            P.begin += Ls > P.LsHi;
-            P.count += Ls > P.LsLo && Ls <= P.LsHi;
+    // Each launch takes the range of that order whose Ls falls in its class.  A batch that
+    // fits one round of the widest class's resident wavefronts runs whole in that launch:
+    // every query then starts at once, and the batch takes as long as its slowest query
+    // rather than the sum of each class's slowest.
+    BatchPart& top = b->parts.back();
+    if (nq <= (uint32_t)top.blocks * top.wavesPerBlock) {
+        for (BatchPart& P : b->parts) P.begin = P.count = 0;
+        top.count = nq;
+    } else {
+        for (BatchPart& P : b->parts) {
+            P.begin = P.count = 0;
+            for (uint32_t i = 0; i < nq; i++) {
+                const int Ls = params[i].Ls;
+                P.begin += Ls > P.LsHi;
+                P.count += Ls > P.LsLo && Ls <= P.LsHi;
+            }
         }
     }
     HIP_TRY(hipMemcpy2DAsync(b->queries, kVecBytes, host_rows, b->dim, b->dim, nq, hipMemcpyHostToDevice, st));
@@ -426,10 +454,18 @@ int pann_batch_launch(pann_batch* b, void* stream) {
     if (!b) return setError(PANN_ERR_ARG, "batch is NULL");
     if (b->nq == 0) return PANN_OK;
     hipError_t e = hipSuccess;
-    // longest class first; a small class launches only as many wavefronts as it has queries
+    const hipStream_t st = (hipStream_t)stream;
+    int used = 0;
+    for (const BatchPart& P : b->parts) used += P.count > 0;
+    // more than one class: the narrower ones fork onto the batch's streams
+    if (used > 1) HIP_TRY(hipEventRecord(b->forkEv, st));
+    // widest class first; a small class launches only as many wavefronts as it has queries
     for (size_t i = b->parts.size(); i-- > 0 && e == hipSuccess;) {
         const BatchPart& P = b->parts[i];
         if (P.count == 0) continue;
+        const bool fork = used > 1 && i + 1 < b->parts.size();
+        const hipStream_t ps = fork ? b->side[i] : st;
+        if (fork) HIP_TRY(hipStreamWaitEvent(ps, b->forkEv, 0));
         const int blocks = (int)std::min<uint32_t>((uint32_t)P.blocks,
                                                    (P.count + P.wavesPerBlock - 1) / P.wavesPerBlock);
         const uint32_t waves = (uint32_t)blocks * P.wavesPerBlock;
@@ -439,16 +475,20 @@ int pann_batch_launch(pann_batch* b, void* stream) {
             L.order = b->order + P.begin;
             L.queue = b->queue + 2 * i;
             L.totalWaves = waves;
-            e = launchBeamFlat(L, b->maxch, P.wavesPerBlock, blocks, (hipStream_t)stream);
+            e = launchBeamFlat(L, b->maxch, P.wavesPerBlock, blocks, ps);
         } else {
             HnswLaunch H = P.H;
             H.nq = P.count;
             H.order = b->order + P.begin;
             H.queue = b->queue + 2 * i;
             H.totalWaves = waves;
-            e = launchBeamHnsw(H, P.wr, b->maxch, P.wavesPerBlock, blocks, (hipStream_t)stream);
+            e = launchBeamHnsw(H, P.wr, b->maxch, P.wavesPerBlock, blocks, ps);
         }
+        if (e == hipSuccess && fork) e = hipEventRecord(b->joinEv[i], ps);
     }
+    // join: the caller's stream waits for every forked launch
+    for (size_t i = 0; e == hipSuccess && used > 1 && i + 1 < b->parts.size(); i++)
+        if (b->parts[i].count) e = hipStreamWaitEvent(st, b->joinEv[i], 0);
     if (e != hipSuccess) return setError(PANN_ERR_HIP, "batched launch: %s", hipGetErrorString(e));
     return PANN_OK;
 }
@@ -522,6 +562,11 @@ void pann_batch_free(pann_batch* b) {
     hipFree(b->queue);
     hipFree(b->gfilt);
     hipFree(b->gvis);
+    for (hipStream_t x : b->side)
+        if (x) hipStreamDestroy(x);
+    for (hipEvent_t x : b->joinEv)
+        if (x) hipEventDestroy(x);
+    if (b->forkEv) hipEventDestroy(b->forkEv);
     freeOutputs(b->out);
     delete b;
 }

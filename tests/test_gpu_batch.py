@@ -71,17 +71,20 @@ def test_hnsw_batch_matches_oracle(gpu):
     rng = np.random.default_rng(2)
     b = search.Vectors(base)
     idx = search.HnswIndex(h, b)
-    batch = search.Batch(len(queries), 300, 100, hnsw=idx)
-    # all four list-width classes (Ls <= 55, <= 119, <= 247, above), then only the widest
-    # and only the narrowest
-    for lo, hi in [(10, 300), (248, 300), (1, 55)]:
-        params = _mixed_params(len(queries), len(base), rng, ls_lo=lo, ls_hi=hi)
-        batch.set_queries(queries, params)
+    nq = 5000  # above the resident wavefronts of every class: one launch per class
+    rows = np.ascontiguousarray(np.resize(queries, (nq, queries.shape[1])))
+    batch = search.Batch(nq, 300, 100, hnsw=idx)
+    assert batch.config()[1] < nq
+    # the list-width classes (Ls <= 55, <= 119, <= 247, <= 311) mixed, only the widest, only
+    # the narrowest, and a batch small enough to run whole in the widest launch
+    for lo, hi, m in [(10, 300, nq), (248, 300, nq), (1, 55, nq), (10, 300, 300)]:
+        params = _mixed_params(m, len(base), rng, ls_lo=lo, ls_hi=hi)
+        batch.set_queries(rows[:m], params)
         batch.launch()
         search.synchronize()
         r = batch.download()
-        o = O.search_hnsw(base, h, queries, _oracle_params(params))
-        _check_against_oracle(r, o, params, f"hnsw batch Ls {lo}..{hi}")
+        o = O.search_hnsw(base, h, rows[:m], _oracle_params(params))
+        _check_against_oracle(r, o, params, f"hnsw batch Ls {lo}..{hi}, {m} queries")
     batch.free(), idx.free(), b.free()
 
 
@@ -183,25 +186,30 @@ def test_register_resident_batch_forms(gpu, w1, monkeypatch, prefetch, wide, max
 
 
 def test_ls_class_launches(gpu, w1):
-    """A flat batch with max_Ls above 64 drains each Ls class (<= 64, 65..128, above 128)
-    in a launch of its own: every class alone, all three mixed, rows wider than 64 entries,
-    and a captured graph that refuses to replay once the class sizes change."""
+    """A flat batch with max_Ls above 64 and more queries than one round of resident
+    wavefronts drains each Ls class (<= 64, 65..128, above 128) in a launch of its own:
+    every class alone, all three mixed, rows wider than 64 entries, and a captured graph
+    that refuses to replay once the class sizes change.  A batch that fits one round runs
+    whole in the widest class's launch."""
     from oracle import oracle as O
     from pann import datagen, search
     from conftest import random_graph
     base, queries, graph = w1
-    nq = 400
+    nq = 5000  # above the 4,096 resident wavefronts of the widest class
+    rows = np.ascontiguousarray(np.resize(queries, (nq, queries.shape[1])))
     rng = np.random.default_rng(31)
     b, g = search.Vectors(base), search.FlatGraph(graph)
     batch = search.Batch(nq, 300, 20, base=b, graph=g)
-    for lo, hi in [(1, 64), (65, 128), (129, 300), (1, 300)]:
-        params = _mixed_params(nq, len(base), rng, ls_lo=lo, ls_hi=hi, ks=(1, 10, 20))
-        batch.set_queries(queries[:nq], params)
+    assert batch.config()[1] < nq
+    for lo, hi, m in [(1, 300, 700), (1, 64, nq), (65, 128, nq), (129, 300, nq), (1, 300, nq)]:
+        params = _mixed_params(m, len(base), rng, ls_lo=lo, ls_hi=hi, ks=(1, 10, 20))
+        batch.set_queries(rows[:m], params)
         batch.launch()
         search.synchronize()
         r = batch.download()
-        o = O.beam_flat(base, graph, queries[:nq], _oracle_params(params))
-        _check_against_oracle(r, o, params, f"Ls {lo}..{hi}")
+        o = O.beam_flat(base, graph, rows[:m], _oracle_params(params))
+        _check_against_oracle(r, o, params, f"Ls {lo}..{hi}, {m} queries")
+    queries = rows
     # the captured graph holds the class ranges of the queries it was captured with
     st = search.Stream()
     batch.capture(st.ptr)
@@ -220,7 +228,7 @@ def test_ls_class_launches(gpu, w1):
     _check_against_oracle(r, o, params, "recaptured")
     batch.free(), st.free(), g.free(), b.free()
     # rows of up to 100 entries: every class runs the general kernel
-    b2, q2 = datagen.gmm_dataset(3000, 200, dim=100, ncomp=6, seeds=(24, 25, 26))
+    b2, q2 = datagen.gmm_dataset(3000, 5000, dim=100, ncomp=6, seeds=(24, 25, 26))
     g2 = random_graph(3000, 100, np.random.default_rng(100), dup_frac=0.1, min_degree=30)
     params = _mixed_params(len(q2), len(b2), rng, ls_lo=5, ls_hi=250, ks=(1, 10, 20))
     b, g = search.Vectors(b2), search.FlatGraph(g2)
