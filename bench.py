@@ -175,7 +175,7 @@ def mixed_workload(queries, nq_total, seed=4):
 
 
 def batched_vs_plain(b, g, rows, params, steps, tune_sample=8192):
-    """QPS of one autotuned persistent launch (HIP-graph replayed) against the plain form:
+    """QPS of the autotuned batched form (its class launches HIP-graph replayed) against the plain form:
     one prepared searcher per distinct (Ls, k, visit), launched one after another as the
     reference's per-configuration protocol does.  Also checks every query agrees."""
     from pann import _lib, search

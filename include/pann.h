@@ -182,7 +182,8 @@ typedef struct pann_batch_config {
     int32_t max_k;           /* output stride */
     int32_t waves_per_block; /* 1, 2 or 4 (0: the most that fit 64 KB of LDS) */
     int32_t lds_filter_bits; /* seen filters up to 2^bits slots live in LDS, 2..12 (0: 10) */
-    uint32_t visited_slots;  /* LDS visited / is_inw hash slots, power of two >= 64 (0: derived) */
+    uint32_t visited_slots;  /* visited / is_inw hash slots per query (LDS, or HBM from 2048 /
+                                4096 slots), power of two >= 64 (0: derived) */
     uint32_t prefetch;       /* next-row prefetch depth, 0 or 1: while a hop merges, load the
                                 row of the predicted next node (flat graphs, max_Ls <= 64) */
 } pann_batch_config;
@@ -209,6 +210,8 @@ int pann_batch_replay(pann_batch* b, void* stream);
 int pann_batch_outputs(const pann_batch* b, pann_search_outputs* out);
 int pann_batch_download(const pann_batch* b, uint32_t* ids, int32_t* dists, int32_t* comps,
                         int32_t* flags, int32_t* edges, int32_t* hops, void* stream);
+/* The configuration as sized (derived fields filled in) and the most wavefronts any one of
+ * the batch's class launches keeps resident. */
 int pann_batch_info(const pann_batch* b, pann_batch_config* cfg, uint32_t* resident_waves);
 void pann_batch_free(pann_batch* b);
 /* Autotuner: times every candidate configuration (wavefronts per block, LDS filter size,
