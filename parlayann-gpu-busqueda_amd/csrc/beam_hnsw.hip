@@ -293,14 +293,15 @@ __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* fi
     __builtin_amdgcn_wave_barrier();
 }
 
-// IG: is_inw in this wave's HBM region (ginw) rather than in LDS.
+// IG: layer 0's is_inw in this wave's HBM region (ginw) rather than in LDS; the upper
+// layers (ef = 1) then keep theirs in LDS, kInwUpperCap slots.
 template <int WR, int MAXCH, bool FGLOBAL, bool IG>
 __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt, uint32_t* ginw,
                         uint32_t q, int Ls, int k, int64_t visit) {
     const int lane = laneId();
     const int sub = lane & 7;
     const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint32_t* inw = IG ? ginw : reinterpret_cast<uint32_t*>(lds + L.lds.inw);
+    uint32_t* ldsInw = reinterpret_cast<uint32_t*>(lds + L.lds.inw);
     uint32_t* ldsFilt = reinterpret_cast<uint32_t*>(lds + L.lds.filt);
     // rows may hold |p|^2 in bytes 124..127 (lane 7's .w): masked out of the products
     const int normMask = (L.normsInRow && sub == 7) ? -1 : 0;
@@ -309,9 +310,8 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
     const int qq = groupSum8(sliceNorm(qv));
     const int64_t lim0 = (visit > 0 && visit < (int64_t)L.n) ? visit : (int64_t)L.n;
     const int bits0 = hnswFilterBits(Ls);
-    uint32_t capInw = hnswInwCap(Ls);
-    if (capInw > L.capInw) capInw = L.capInw;
-    const uint32_t logCap = (uint32_t)ceilLog2(capInw);
+    uint32_t capInw0 = hnswInwCap(Ls);
+    if (capInw0 > L.capInw) capInw0 = L.capInw;
 
     uint32_t eps[8];
     int numEps = L.numEntries;
@@ -331,6 +331,10 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
         uint32_t* filt = globalFilter ? gfilt : ldsFilt;
         if (globalFilter) filtClear<true>(filt, bits, lane);
         else filtClear<false>(filt, bits, lane);
+        const bool hbmInw = IG && layer == 0;
+        uint32_t* inw = hbmInw ? ginw : ldsInw;
+        const uint32_t capInw = (IG && layer > 0) ? min(capInw0, kInwUpperCap) : capInw0;
+        const uint32_t logCap = (uint32_t)ceilLog2(capInw);
         {
             const uint4 e4 = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
             uint4* i4 = reinterpret_cast<uint4*>(inw);
@@ -347,7 +351,9 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
                 const uint32_t slot = (uint32_t)hash64(pe) & fmask;
                 if (globalFilter) filtStore<true>(filt, slot, pe);
                 else filtStore<false>(filt, slot, pe);
-                if (inwInsertOne<IG>(inw, pe, logCap, capInw - 1)) st.inwCount++;
+                const bool added = hbmInw ? inwInsertOne<true>(inw, pe, logCap, capInw - 1)
+                                          : inwInsertOne<false>(inw, pe, logCap, capInw - 1);
+                if (added) st.inwCount++;
             }
             if (globalFilter) __builtin_amdgcn_s_waitcnt(0);
             __builtin_amdgcn_wave_barrier();
@@ -368,12 +374,21 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
             if (++cnt > limit) break;
             cand.popFront(lane);
             st.hops++;
-            if (globalFilter)
-                expandNode<WR, MAXCH, true, IG>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq, W,
-                                            cand, st, inw, capInw, logCap, lane, lanesBelow);
-            else
-                expandNode<WR, MAXCH, false, IG>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq,
-                                             W, cand, st, inw, capInw, logCap, lane, lanesBelow);
+            if (globalFilter) {
+                if (hbmInw)
+                    expandNode<WR, MAXCH, true, IG>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq,
+                                                    W, cand, st, inw, capInw, logCap, lane, lanesBelow);
+                else
+                    expandNode<WR, MAXCH, true, false>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq,
+                                                       W, cand, st, inw, capInw, logCap, lane, lanesBelow);
+            } else {
+                if (hbmInw)
+                    expandNode<WR, MAXCH, false, IG>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq,
+                                                     W, cand, st, inw, capInw, logCap, lane, lanesBelow);
+                else
+                    expandNode<WR, MAXCH, false, false>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv,
+                                                        qq, W, cand, st, inw, capInw, logCap, lane, lanesBelow);
+            }
             if (st.inwCount + 64u * MAXCH >= capInw) {
                 st.flags |= kFlagOverflow;
                 break;

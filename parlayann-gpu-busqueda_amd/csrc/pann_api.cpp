@@ -569,7 +569,7 @@ int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_searc
     const bool ldsFilter = bits0 <= kFilterBitsLds;
     const bool inwHbm = wr >= 2 && hnswInwCap(Ls) >= kInwHbmMinCap;
     const uint32_t capInw = inwHbm ? hnswInwCap(Ls) : hnswInwCapFit(Ls, maxch, ldsFilter ? bits0 : 0, kMaxLdsPerBlock);
-    const HnswLds lay = hnswLdsLayout(maxch, inwHbm ? 0 : capInw, ldsFilter ? bits0 : 0);
+    const HnswLds lay = hnswLdsLayout(maxch, inwHbm ? kInwUpperCap : capInw, ldsFilter ? bits0 : 0);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "Ls=%d needs %u bytes of LDS per query", Ls, lay.total);
 

@@ -205,7 +205,7 @@ int planHnswPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int*
     // a launch whose shortest beam already needs a larger layer-0 filter keeps only the
     // 4-slot upper-layer filter in LDS
     const int fbLaunch = hnswFilterBits(LsLo + 1) > fbLds ? 2 : fbLds;
-    const HnswLds lay = hnswLdsLayout(b->maxch, inwHbm ? 0 : capInw, fbLaunch);
+    const HnswLds lay = hnswLdsLayout(b->maxch, inwHbm ? kInwUpperCap : capInw, fbLaunch);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsHi, lay.total);
     P->LsLo = LsLo;

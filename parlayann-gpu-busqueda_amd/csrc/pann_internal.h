@@ -180,9 +180,11 @@ inline int hnswListWidth(int Ls, uint32_t numEntries) {
     return need <= 64 ? 1 : need <= 128 ? 2 : need <= 256 ? 4 : need <= 320 ? 5 : need <= 512 ? 8 : 0;
 }
 
-// Launches whose is_inw set has at least this many slots (ef > 112) keep it in HBM
-// (HnswLaunch::ginw): in LDS it would leave one wavefront per SIMD at ef around 256.
+// Launches whose is_inw set has at least this many slots (ef > 112) keep layer 0's in HBM
+// (HnswLaunch::ginw): in LDS it would leave one wavefront per SIMD at ef around 256.  Their
+// upper layers (ef = 1) keep a kInwUpperCap-slot set in LDS, the layout's inw region.
 constexpr uint32_t kInwHbmMinCap = 4096;
+constexpr uint32_t kInwUpperCap = 1024;
 
 struct HnswLds {
     uint32_t pend;   // pending ids (64*maxch u32)
