@@ -508,6 +508,9 @@ hipError_t occT(bool inwHbm, int wpb, size_t lds, int* bpc) {
         case 21: return FN<2, 1>(__VA_ARGS__);                                \
         case 22: return FN<2, 2>(__VA_ARGS__);                                \
         case 24: return FN<2, 4>(__VA_ARGS__);                                \
+        case 31: return FN<3, 1>(__VA_ARGS__);                                \
+        case 32: return FN<3, 2>(__VA_ARGS__);                                \
+        case 34: return FN<3, 4>(__VA_ARGS__);                                \
         case 41: return FN<4, 1>(__VA_ARGS__);                                \
         case 42: return FN<4, 2>(__VA_ARGS__);                                \
         case 44: return FN<4, 4>(__VA_ARGS__);                                \
