@@ -199,8 +199,10 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
 int pann_batch_set_queries(pann_batch* b, const int8_t* host_rows, const pann_query_params* params,
                            uint32_t nq, void* stream);
 /* Persistent launches whose resident wavefronts drain the queue: one per Ls class present
- * (flat graphs: Ls <= 64, 65..128, above 128), each sized for its class, one after another
- * on `stream`. */
+ * (flat graphs: Ls <= 64, 65..128, above 128; HNSW: Ls <= 55, 119, 183, 247, 311, above),
+ * each sized for its class; the narrower classes run on the batch's own streams, forked
+ * from and joined back to `stream`.  A batch that fits one round of the widest class's
+ * resident wavefronts runs whole in that launch. */
 int pann_batch_launch(pann_batch* b, void* stream);
 /* Records those launches into a HIP graph / replays it.  The graph holds the class ranges
  * of the queries set at capture time: replay fails once set_queries changed the number of
