@@ -99,11 +99,23 @@ def test_dim128_and_small_dims(gpu):
 
 
 def test_cut_values(gpu, small):
+    """The cut at (float)(cut * F[k].d), cut below and above 1, in every form: one key per
+    lane (Ls 40), two (Ls 100), the general form with the visited table in HBM (Ls 200) and
+    the general form on rows wider than 64 entries."""
+    from pann import datagen
     base, queries, graph = small
-    for cut in (1.0, 1.1, 2.0, 10.0):
-        r = _run(base, graph, queries, 40, 10, cut=cut)
-        o = _oracle(base, graph, queries, 40, 10, cut=cut)
-        _assert_same(r, o, 10, f"cut={cut}")
+    for Ls in (40, 100, 200):
+        for cut in (0.9, 1.0, 1.1, 2.0, 10.0):
+            r = _run(base, graph, queries, Ls, 10, cut=cut)
+            o = _oracle(base, graph, queries, Ls, 10, cut=cut)
+            _assert_same(r, o, 10, f"Ls={Ls} cut={cut}")
+    b2, q2 = datagen.gmm_dataset(3000, 150, dim=100, ncomp=6, seeds=(27, 28, 29))
+    g2 = random_graph(3000, 100, np.random.default_rng(7), dup_frac=0.1, min_degree=30)
+    for Ls in (30, 150):
+        for cut in (0.9, 1.1):
+            r = _run(b2, g2, q2, Ls, 10, cut=cut)
+            o = _oracle(b2, g2, q2, Ls, 10, cut=cut)
+            _assert_same(r, o, 10, f"G=100 Ls={Ls} cut={cut}")
 
 
 def test_determinism_repeat_launch(gpu, small):
