@@ -118,7 +118,7 @@ def test_oracle_matches_python_restatement(seed):
     stats = {"overwrites": 0}
     for Ls, k, visit, cut in [(1, 1, None, 1.35), (7, 5, None, 1.35), (20, 10, None, 1.35),
                               (33, 10, 9, 1.35), (50, 20, None, 1.0), (64, 64, None, 2.0),
-                              (5, 10, None, 1.35)]:
+                              (5, 10, None, 1.35), (100, 10, None, 0.9), (150, 10, 60, 1.35)]:
         o = _oracle(base, graph, queries, qids, Ls, k, visit, cut)
         for i in range(nq):
             ids, ds, comps, hops, ed, flags = py_beam_search(
