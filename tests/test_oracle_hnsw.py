@@ -117,7 +117,8 @@ def test_oracle_matches_python_restatement(seed):
                     max_level=3, num_entries=int(rng.integers(1, 5)), dup_frac=0.2,
                     relabel=bool(seed % 2), base_n=n + 10)
     queries = rng.integers(-25, 25, size=(10, dim)).astype(np.int8)
-    for Ls, k, visit in [(1, 1, None), (4, 3, None), (12, 10, None), (40, 10, 7), (70, 20, None)]:
+    for Ls, k, visit in [(1, 1, None), (4, 3, None), (12, 10, None), (40, 10, 7), (70, 20, None),
+                         (150, 10, None), (260, 10, 80)]:
         o = O.search_hnsw(base, h, queries, O.make_params(10, Ls, k, visit))
         for i in range(10):
             ids, ds, comps, hops, edges = py_hnsw(base, h, queries[i], Ls, k, visit)
