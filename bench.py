@@ -30,6 +30,7 @@ import numpy as np  # noqa: E402
 METRIC = ("QPS @ recall@10=0.90, 1M×100 int8; batched vs plain on 1 GPU; p99 ms")
 LS_SWEEP = [10, 12, 15, 20, 25, 30, 35, 40, 50, 60, 75, 100, 125, 150, 200, 300, 400, 500]
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SOURCES = ("beam_flat.hip", "device_common.h", "pann_internal.h")
 
 
 def log(*a):
@@ -91,6 +92,131 @@ def time_launches(searcher, b, q, g, steps, warmup):
     for e in evs:
         L.pann_event_destroy(e)
     return wall, tot / steps
+
+
+def end_to_end(searcher, b, q, g, queries, reps=5):
+    """The reference's end-to-end protocol (cuda/comparacion1.cu:67-75, 96-109): query rows
+    from pinned host memory to the device (pann_vectors_copy_async, which also refreshes the
+    norms), the launch, ids and distances back to pinned host memory, stream synchronize;
+    one warm-up then best of `reps` by the host's steady clock.  Determinism: the ids of the
+    warm-up run equal those of the last run."""
+    from pann import _lib, search
+    import ctypes as C
+    L = _lib.lib()
+    nq, dim = queries.shape
+    k = searcher.cfg.k
+    ptrs = []
+
+    def pinned(nbytes, ctype, count):
+        p = C.c_void_p()
+        _lib.check(L.pann_host_alloc(nbytes, C.byref(p)))
+        ptrs.append(p.value)
+        return np.ctypeslib.as_array((ctype * count).from_address(p.value))
+    rows = pinned(nq * dim, C.c_int8, nq * dim).reshape(nq, dim)
+    rows[:] = queries
+    ids = pinned(nq * k * 4, C.c_uint32, nq * k)
+    dists = pinned(nq * k * 4, C.c_int32, nq * k)
+    st = search.Stream()
+    rows_p = rows.ctypes.data_as(C.c_void_p)
+    ids_p, dists_p = ids.ctypes.data_as(C.c_void_p), dists.ctypes.data_as(C.c_void_p)
+
+    def run():
+        _lib.check(L.pann_vectors_copy_async(q.handle, rows_p, nq, st.ptr))
+        searcher.launch(b, q, g, stream=st.ptr)
+        _lib.check(L.pann_searcher_download(searcher.handle, ids_p, dists_p, None, None, None, None,
+                                            st.ptr))
+        _lib.check(L.pann_synchronize(st.ptr))
+    run()
+    warm = ids.copy()
+    best = float("inf")
+    for _ in range(reps):
+        t = time.perf_counter()
+        run()
+        best = min(best, time.perf_counter() - t)
+    deterministic = bool(np.array_equal(ids, warm))
+    last = ids.copy().reshape(nq, k)
+    st.free()
+    for p in ptrs:
+        L.pann_host_free(p)
+    return {"e2e_qps": round(nq / best, 1), "e2e_ms": round(best * 1e3, 4), "reps": reps,
+            "deterministic": deterministic}, last
+
+
+def gather_ceiling(b, q, count=1 << 26, big_rows=8 << 20, seed=1):
+    """Random 128-byte row gathers with v_dot4 (pann_gather_bench) over the 1M-row base
+    (128 MB) and over a table of big_rows random rows (1 GiB at the default), GB/s counted
+    as 128 bytes per distance; the chip's 256 MB last-level cache holds the first table,
+    not the second."""
+    from pann import search
+    out = {}
+    ms, _, gbs = search.gather_bench(b, q, count, seed=seed, reps=5)
+    out[f"table_{b.n * 128 >> 20}MiB"] = {"gbs": round(gbs, 1), "ms": round(ms, 4), "count": count}
+    rng = np.random.Generator(np.random.PCG64(seed))
+    big = search.Vectors(rng.integers(-128, 128, size=(big_rows, 128), dtype=np.int8))
+    try:
+        ms, _, gbs = search.gather_bench(big, q, count, seed=seed, reps=5)
+    finally:
+        big.free()
+    out[f"table_{big_rows * 128 >> 20}MiB"] = {"gbs": round(gbs, 1), "ms": round(ms, 4), "count": count}
+    return out
+
+
+def kernel_sources_sha():
+    """sha256 over the sources of the flat search kernel: a stored traffic record applies
+    only to the kernel these sources build."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in TRAFFIC_SOURCES:
+        with open(os.path.join(PKG, "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def stored_traffic(path, Ls, n):
+    """HBM-side bytes per launch from the committed counter record, or (None, why)."""
+    if not os.path.exists(path):
+        return None, "no counter record"
+    tj = json.load(open(path))
+    if tj.get("workload_Ls") != Ls or tj.get("n") != n:
+        return None, f"counter record is for Ls={tj.get('workload_Ls')} n={tj.get('n')}"
+    if tj.get("sources_sha256") != kernel_sources_sha():
+        return None, "counter record predates the current kernel sources"
+    return tj.get("hbm_bytes_per_launch"), (f"rocprofv3 FETCH_SIZE/WRITE_SIZE of {tj.get('kernel')} "
+                                            f"scaled by the copy kernel's counters")
+
+
+def write_traffic(args):
+    """Builds the counter record from four rocprofv3 --pmc CSV files: FETCH_SIZE and
+    WRITE_SIZE of the search kernel, then of the 1 GiB copy kernel (its true traffic is
+    1 GiB read + 1 GiB written, which gives the scale factors)."""
+    import csv
+    import re
+
+    def mean_counter(path, kernel_re, counter):
+        vals = []
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") == counter and re.search(kernel_re, row.get("Kernel_Name", "")):
+                    vals.append(float(row["Counter_Value"]))
+        if not vals:
+            raise SystemExit(f"no {counter} rows for /{kernel_re}/ in {path}")
+        return sum(vals) / len(vals), len(vals)
+    f_csv, w_csv, cf_csv, cw_csv = args.write_traffic
+    kre = args.traffic_kernel
+    fetch, nf = mean_counter(f_csv, kre, "FETCH_SIZE")
+    write, nw = mean_counter(w_csv, kre, "WRITE_SIZE")
+    cfetch, _ = mean_counter(cf_csv, "copyKernel", "FETCH_SIZE")
+    cwrite, _ = mean_counter(cw_csv, "copyKernel", "WRITE_SIZE")
+    gib_kib = float(1 << 20)
+    fs, ws = gib_kib / cfetch, gib_kib / cwrite
+    rec = {"kernel": kre, "workload_Ls": args.ls, "n": args.n, "nq": args.nq,
+           "fetch_kib": round(fetch, 3), "write_kib": round(write, 3), "dispatches": [nf, nw],
+           "fetch_scale": fs, "write_scale": ws,
+           "hbm_bytes_per_launch": int(round((fetch * fs + write * ws) * 1024)),
+           "sources_sha256": kernel_sources_sha()}
+    with open(args.traffic_json, "w") as f:
+        json.dump(rec, f, indent=1)
+    print(json.dumps(rec))
 
 
 def cpu_baseline(base, graph, queries, Ls, k, seconds):
@@ -474,9 +600,21 @@ def main():
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run the copy kernel over this many bytes (counter calibration)")
+    ap.add_argument("--write-traffic", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV",
+                                                         "COPY_WRITE_CSV"),
+                    help="only write the counter record (--traffic-json) from rocprofv3 CSVs")
+    ap.add_argument("--traffic-kernel", default="beamFlatFastKernel",
+                    help="kernel-name regex of the counter record")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="headline: time the plain launches only (plain-only kernel traces)")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end (H2D+search+D2H) leg")
+    ap.add_argument("--no-gather", action="store_true", help="skip the random-gather ceiling")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy, 3)
+        return
+    if args.write_traffic:
+        write_traffic(args)
         return
 
     if args.gpus != 1:
@@ -529,26 +667,36 @@ def main():
     plain_wall, plain_ev = time_launches(s, b, q, g, args.steps, args.warmup)
     log(f"[bench] plain Ls={Ls}: {plain_wall:.3f} ms/launch wall, {plain_ev:.3f} ms kernel, "
         f"{args.nq / plain_wall * 1e3:,.0f} QPS")
-    bat_wall, bat_ev, rbat, bcfg, bwaves, btune = homogeneous_batched(b, g, queries, Ls, k, args.steps,
-                                                                       args.warmup, not args.no_tune)
-    same = bool(np.array_equal(rbat.ids[:, :k], r.ids) and np.array_equal(rbat.comps, r.comps))
-    log(f"[bench] batched Ls={Ls}: {bat_wall:.3f} ms/launch wall, {bat_ev:.3f} ms kernel, "
-        f"{args.nq / bat_wall * 1e3:,.0f} QPS, identical to plain: {same}")
-    form = "batched persistent (HIP graph replay)" if (bat_wall < plain_wall and same) else "plain"
+    e2e = None
+    if not args.no_e2e:
+        e2e, last_ids = end_to_end(s, b, q, g, queries)
+        e2e["identical_to_first_launch"] = bool(np.array_equal(last_ids, r.ids))
+        log(f"[bench] end to end Ls={Ls}: {e2e['e2e_ms']:.3f} ms best of {e2e['reps']}, "
+            f"{e2e['e2e_qps']:,.0f} QPS, deterministic {e2e['deterministic']}, "
+            f"same as the first launch {e2e['identical_to_first_launch']}")
+    if args.no_batched:
+        bat_wall, bat_ev, rbat, bcfg, bwaves, btune = None, None, None, None, None, 0.0
+        same = None
+    else:
+        bat_wall, bat_ev, rbat, bcfg, bwaves, btune = homogeneous_batched(b, g, queries, Ls, k, args.steps,
+                                                                           args.warmup, not args.no_tune)
+        same = bool(np.array_equal(rbat.ids[:, :k], r.ids) and np.array_equal(rbat.comps, r.comps))
+        log(f"[bench] batched Ls={Ls}: {bat_wall:.3f} ms/launch wall, {bat_ev:.3f} ms kernel, "
+            f"{args.nq / bat_wall * 1e3:,.0f} QPS, identical to plain: {same}")
+    form = ("batched persistent (HIP graph replay)"
+            if (bat_wall is not None and bat_wall < plain_wall and same) else "plain")
     wall_ms, ev_ms = (bat_wall, bat_ev) if form != "plain" else (plain_wall, plain_ev)
     qps = args.nq / (wall_ms / 1e3)
     alg_bytes = float(100 * r.comps.astype(np.float64).sum() + 4 * r.edges.astype(np.float64).sum())
     achieved = alg_bytes / (ev_ms / 1e3) / 1e9
-    traffic = None
-    if os.path.exists(args.traffic_json):
-        try:
-            tj = json.load(open(args.traffic_json))
-            if tj.get("workload_Ls") == Ls and tj.get("n") == args.n:
-                traffic = tj.get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+    traffic, traffic_note = stored_traffic(args.traffic_json, Ls, args.n)
     log(f"[bench] Ls={Ls}: {wall_ms:.3f} ms/launch wall, {ev_ms:.3f} ms kernel, {qps:,.0f} QPS, "
-        f"{achieved:.1f} GB/s algorithmic")
+        f"{achieved:.1f} GB/s algorithmic; traffic: {traffic_note}")
+    gather = None
+    if not args.no_gather:
+        gather = gather_ceiling(b, q)
+        log(f"[bench] random-gather ceiling (128 B rows, v_dot4): "
+            + ", ".join(f"{key} {v['gbs']:,.0f} GB/s" for key, v in gather.items()))
 
     mixed = None
     if not args.no_mixed:
@@ -600,13 +748,14 @@ def main():
             "form": form, "graph": "vamana", "n": args.n, "nq": args.nq, "Ls": Ls,
             "recall@10": round(rec, 5),
             "plain_qps": round(args.nq / plain_wall * 1e3, 1), "plain_kernel_ms": round(plain_ev, 4),
-            "batched_qps": round(args.nq / bat_wall * 1e3, 1), "batched_kernel_ms": round(bat_ev, 4),
+            "end_to_end": e2e,
+            "batched_qps": round(args.nq / bat_wall * 1e3, 1) if bat_wall else None,
+            "batched_kernel_ms": round(bat_ev, 4) if bat_ev else None,
             "batched_identical_to_plain": same,
-            "batched_shape": {"waves_per_block": bcfg["waves_per_block"],
-                              "lds_filter_bits": bcfg["lds_filter_bits"],
-                              "visited_slots": bcfg["visited_slots"], "prefetch": bcfg["prefetch"],
-                              "resident_waves": bwaves, "autotuned": not args.no_tune,
-                              "autotune_s": round(btune, 2)},
+            "batched_shape": None if bcfg is None else {
+                "waves_per_block": bcfg["waves_per_block"], "lds_filter_bits": bcfg["lds_filter_bits"],
+                "visited_slots": bcfg["visited_slots"], "prefetch": bcfg["prefetch"],
+                "resident_waves": bwaves, "autotuned": not args.no_tune, "autotune_s": round(btune, 2)},
             "kernel_ms": round(ev_ms, 4), "comps_per_query": round(float(r.comps.mean()), 1),
             "edges_per_query": round(float(r.edges.mean()), 1),
             "hops_per_query": round(float(r.hops.mean()), 1),
@@ -614,7 +763,9 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+                     "traffic": traffic, "traffic_source": traffic_note,
+                     "algorithmic_bytes_per_launch": int(alg_bytes),
+                     "gather_ceiling": gather},
         "cpu_baseline": cpu,
         "batched_vs_plain": mixed,
         "serving": serve,

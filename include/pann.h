@@ -65,8 +65,9 @@ typedef struct pann_query_params {
 /* Device output buffers owned by a searcher (idsDev, distsDev, compDev, banderasDev,
  * cuda/busqueda.cuh:42-46) plus adjacency entries read and hops per query. */
 typedef struct pann_search_outputs {
-    uint32_t* ids;    /* nq x stride */
-    int32_t* dists;   /* nq x stride */
+    uint32_t* ids;    /* nq x stride; columns past a query's k (or its frontier) hold
+                         PANN_NO_NEIGHBOR */
+    int32_t* dists;   /* nq x stride; INT32_MAX where ids hold PANN_NO_NEIGHBOR */
     int32_t* comps;   /* nq: distance computations (ParlayANN dist_cmps) */
     int32_t* flags;   /* nq: PANN_FLAG_* */
     int32_t* edges;   /* nq: adjacency entries read, padding excluded */
@@ -99,6 +100,15 @@ void pann_event_destroy(void* ev);
 /* Plain streaming copy kernel (16-byte vector loads/stores, bytes % 16 == 0): a launch of
  * known traffic used to scale hardware byte counters. */
 int pann_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
+/* Random-gather ceiling of 128-byte rows, the reference's microbenchmark
+ * (cuda/micro_distancias.cu:41-63, eight lanes per row, variant C of :65-132): `count`
+ * distances between the row chosen by a hash of (seed + i) and query row i mod 2^m
+ * (2^m the largest power of two <= queries' rows), one v_dot4_i32_i8 per dword.  One
+ * warm-up launch, then `reps` timed launches on the null stream; *ms is the mean launch
+ * time, *checksum the sum of every q.p of a launch (all 128 bytes of each row).  Blocking.
+ * Gathered bytes per launch = 128 * count. */
+int pann_gather_bench(const pann_vectors* rows, const pann_vectors* queries, uint64_t count,
+                      uint64_t seed, int reps, float* ms, uint64_t* checksum);
 /* Per-section cycle sums of the flat search, as the reference's -DPERFIL build keeps
  * them (cuda/busqueda.cu:15-32): init, visit, filter, distances, sort, union, merge,
  * cut, next, output, then the hop and query counts, then (register-resident form) the
@@ -265,14 +275,17 @@ void pann_server_free(pann_server* s);
 
 /* Open-loop load generator: Poisson arrivals of `rate_qps` queries per second for
  * duration_s, request sizes log-uniform in [1, max_request], per-query Ls log-uniform in
- * [Ls_lo, Ls_hi], k drawn from ks[nks], qid = row index into rows[nrows].  Fills *stats;
- * when record_ids is not NULL, the first min(k, 10) ids of up to record_cap served queries
- * are written there (10 per query) with their row indices in record_rows. */
+ * [Ls_lo, Ls_hi], k drawn from ks[nks], qid = row index into rows[nrows].  Fills *stats.
+ * Up to record_cap served queries are recorded in service order into the non-NULL record
+ * arrays: the first min(k, 10) ids and distances (10 columns per query, the rest
+ * PANN_NO_NEIGHBOR / INT32_MAX), the flags, the row index and the query's parameters. */
 int pann_server_run_poisson(pann_server* s, const int8_t* rows, uint32_t nrows, double rate_qps,
                             double duration_s, uint32_t max_request, int32_t Ls_lo, int32_t Ls_hi,
                             const int32_t* ks, uint32_t nks, uint64_t seed,
-                            pann_server_stats* stats, uint32_t* record_ids, uint32_t* record_rows,
-                            uint32_t record_cap, uint32_t* recorded);
+                            pann_server_stats* stats, uint32_t* record_ids, int32_t* record_dists,
+                            int32_t* record_flags, uint32_t* record_rows,
+                            pann_query_params* record_params, uint32_t record_cap,
+                            uint32_t* recorded);
 
 /* ---- exact ground truth ---- */
 /* Exact top-k (k <= 128) of the first nq queries over all base rows, ordered by

@@ -7,12 +7,12 @@
  * Every function names the reference lines it follows and what pins it:
  *
  *   oracle_hash64_2   cuda/utiles_gpu.cuh:12-17   parlaylib hash64_2 (SplitMix64
- *                     finalizer).  Pinned by the published SplitMix64 output stream
- *                     (tests/test_oracle_hash.py).
+ *                     finalizer).  Pinned by the published SplitMix64 output stream and
+ *                     by known answers evaluated from the reference header's constants
+ *                     (tests/test_oracle_hash.py, tests/golden/hash_vectors.json).
  *   oracle_hash64     cuda/utiles_gpu.cuh:21-31   parlaylib hash64 (Numerical Recipes
- *                     3rd ed. Ranhash int64).  Pinned by invertibility of each step
- *                     (bijection check on random inputs) and by the HNSW entry-point
- *                     special case in tests/test_oracle_hnsw.py.
+ *                     3rd ed. Ranhash int64).  Pinned by the same known answers and a
+ *                     no-collision check on a sample (tests/test_oracle_hash.py).
  *   oracle_dist_i8    cuda/lectores.cpp:172-179, cuda/distancia.cuh:27-58
  *                     exact int32 squared L2.  Pinned by numpy brute force.
  *   oracle_beam_flat  cuda/busqueda.cu:95-506 (ParlayANN beam_search: lossy seen
@@ -25,6 +25,9 @@
  *                     ef = n) and a pure-Python restatement on tiny graphs.  The
  *                     workset tie order of upstream ParlayANN (a std heap) is NOT
  *                     reproduced: this oracle breaks ties by (distance, id).
+ *   oracle_bruteforce_knn  exact top-k by (distance, id).  Pinned by a numpy int64
+ *                     brute force with lexsort, ties and k = n included
+ *                     (tests/test_oracle_bruteforce.py).
  *   oracle_recall     cuda/evaluacion.cpp:56-80 (big-ann-benchmarks
  *                     compute_recall_with_distance_ties).  Pinned by hand-computed
  *                     cases.

@@ -436,8 +436,8 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
     if (fsize < c.k) flags |= kFlagShort;
     uint32_t* oi = L.outIds + (size_t)q * L.outStride;
     int32_t* od = L.outDists + (size_t)q * L.outStride;
-    for (int j = lane; j < c.k; j += 64) {
-        if (j < fsize) {
+    for (int j = lane; j < (int)L.outStride; j += 64) {
+        if (j < c.k && j < fsize) {
             const uint64_t kv = F[j];
             oi[j] = (uint32_t)kv;
             od[j] = (int32_t)(kv >> 32);
@@ -860,8 +860,8 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
     if (fsize < c.k) flags |= kFlagShort;
     uint32_t* oi = L.outIds + (size_t)q * L.outStride;
     int32_t* od = L.outDists + (size_t)q * L.outStride;
-    for (int j = lane; j < c.k; j += 64) {
-        if (j < fsize) {
+    for (int j = lane; j < (int)L.outStride; j += 64) {
+        if (j < c.k && j < fsize) {
             oi[j] = keyId(F);
             od[j] = keyDist(F);
         } else {
@@ -1157,8 +1157,8 @@ __device__ void searchFast2(const FlatLaunch& L, unsigned char* lds, uint32_t q,
     if (fsize < c.k) flags |= kFlagShort;
     uint32_t* oi = L.outIds + (size_t)q * L.outStride;
     int32_t* od = L.outDists + (size_t)q * L.outStride;
-    for (int j = lane; j < c.k; j += 64) {
-        if (j < fsize) {
+    for (int j = lane; j < (int)L.outStride; j += 64) {
+        if (j < c.k && j < fsize) {
             const uint64_t kv = j < 64 ? F0 : F1;
             oi[j] = keyId(kv);
             od[j] = keyDist(kv);
@@ -1221,20 +1221,20 @@ __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
         const int fb = flatFilterBits(c.Ls);
         if constexpr (FAST >= 9) {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride || fb > L.gbitsMax) {
-                if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
+                rejectQuery(L, q, lane);
             } else {
                 searchFast2<((FAST - 9) & 1) != 0, ((FAST - 9) & 2) != 0>(L, lds, q, c, gfilt, P);
             }
         } else if constexpr (FAST > 0) {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride) {
-                if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
+                rejectQuery(L, q, lane);
             } else {
                 searchFast<((FAST - 1) & 1) != 0, ((FAST - 1) & 2) != 0, ((FAST - 1) & 4) != 0>(L, lds, q, c, P);
             }
         } else {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride ||
                 (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
-                if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
+                rejectQuery(L, q, lane);
             } else if (!VG && MAXCH == 1 && c.Ls <= kFastLsMax && L.fbitsLds >= kFastFilterBits) {
                 if (L.normsInRow) searchFast<false, false, true>(L, lds, q, c, P);
                 else searchFast<false, false, false>(L, lds, q, c, P);

@@ -407,8 +407,8 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
 #pragma unroll
     for (int r = 0; r < WR; r++) {
         const int e = r * 64 + lane;
-        if (e < k) {
-            if (e < W.size) {
+        if (e < (int)L.outStride) {
+            if (e < k && e < W.size) {
                 oi[e] = L.nodePoint[(uint32_t)W.v[r]];
                 od[e] = (int32_t)(W.v[r] >> 32);
             } else {
@@ -417,7 +417,7 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
             }
         }
     }
-    for (int e = WR * 64 + lane; e < k; e += 64) {
+    for (int e = WR * 64 + lane; e < (int)L.outStride; e += 64) {
         oi[e] = kEmpty;
         od[e] = 0x7FFFFFFF;
     }
@@ -461,7 +461,7 @@ void beamHnswKernel(HnswLaunch L) {
         const int fb = hnswFilterBits(Ls);
         if (Ls < 1 || Ls > L.LsMax || k < 0 || (uint32_t)k > L.outStride ||
             (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
-            if (lane == 0 && L.outFlags) L.outFlags[q] = kFlagConfig;
+            rejectQuery(L, q, lane);
         } else if (fb <= L.fbitsLds) {
             hnswOne<WR, MAXCH, false, IG>(L, lds, nullptr, ginw, q, Ls, k, visit);
         } else {

@@ -63,25 +63,6 @@ __device__ __forceinline__ void bitonicSort(uint64_t (&key)[C], int lane) {
     }
 }
 
-__global__ __launch_bounds__(256) void rowNormsKernel(int8_t* rows, uint32_t n, int32_t* out,
-                                                     bool embed) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int4* r = reinterpret_cast<int4*>(rows + (size_t)i * kVecBytes);
-    int s = 0;
-#pragma unroll
-    for (int j = 0; j < kVecBytes / 16; j++) {
-        int4 v = r[j];
-        if (embed && j == kVecBytes / 16 - 1) v.w = 0;  // the norm's own bytes
-        s = __builtin_amdgcn_sdot4(v.x, v.x, s, false);
-        s = __builtin_amdgcn_sdot4(v.y, v.y, s, false);
-        s = __builtin_amdgcn_sdot4(v.z, v.z, s, false);
-        s = __builtin_amdgcn_sdot4(v.w, v.w, s, false);
-    }
-    out[i] = s;
-    if (embed) reinterpret_cast<int32_t*>(rows + (size_t)i * kVecBytes)[kVecBytes / 4 - 1] = s;
-}
-
 struct BfLds {
     uint64_t top[kQ][kTopCap];
     uint64_t buf[kQ][kBufCap];
@@ -206,29 +187,7 @@ __global__ __launch_bounds__(64) void bfKnnKernel(const int8_t* base, const int3
     }
 }
 
-__global__ __launch_bounds__(256) void copyKernel(const int4* __restrict__ src,
-                                                  int4* __restrict__ dst, size_t n16) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16;
-         i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
-}
-
 }  // namespace
-
-hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s) {
-    const size_t n16 = bytes / 16;
-    if (n16 == 0) return hipSuccess;
-    const size_t want = (n16 + 255) / 256, blocks = want < 256 * 64 ? want : 256 * 64;
-    hipLaunchKernelGGL(copyKernel, dim3((unsigned)blocks), dim3(256), 0, s,
-                       reinterpret_cast<const int4*>(src), reinterpret_cast<int4*>(dst), n16);
-    return hipGetLastError();
-}
-
-hipError_t launchRowNorms(int8_t* rows, uint32_t n, int32_t* out, bool embed, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(rowNormsKernel, dim3((n + 255) / 256), dim3(256), 0, s, rows, n, out, embed);
-    return hipGetLastError();
-}
 
 hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_t n,
                                const int8_t* queries, const int32_t* qnorm, uint32_t nq,

@@ -410,6 +410,24 @@ __device__ __forceinline__ bool seenFilterChunk(uint32_t* filt, unsigned char* t
     return seenFilterEnd<G>(filt, s, a, part, lane, lanesBelow);
 }
 
+// A query whose parameters fall outside what the launch was sized for: every output
+// column holds the missing-neighbour sentinel, the counters are zero and the flag says why.
+template <class Launch>
+__device__ __forceinline__ void rejectQuery(const Launch& L, uint32_t q, int lane) {
+    uint32_t* oi = L.outIds + (size_t)q * L.outStride;
+    int32_t* od = L.outDists + (size_t)q * L.outStride;
+    for (uint32_t j = lane; j < L.outStride; j += 64) {
+        oi[j] = kEmpty;
+        od[j] = 0x7FFFFFFF;
+    }
+    if (lane == 0) {
+        if (L.outComps) L.outComps[q] = 0;
+        if (L.outFlags) L.outFlags[q] = kFlagConfig;
+        if (L.outEdges) L.outEdges[q] = 0;
+        if (L.outHops) L.outHops[q] = 0;
+    }
+}
+
 // What takeQuery returns once the queue is drained.  The order array may hold a range of
 // a larger batch, so a query index says nothing about the queue's end.
 constexpr uint32_t kQueueEmpty = 0xFFFFFFFFu;

@@ -186,6 +186,43 @@ int pann_copy_kernel(void* dst, const void* src, size_t bytes, void* stream) {
     return PANN_OK;
 }
 
+int pann_gather_bench(const pann_vectors* rows, const pann_vectors* queries, uint64_t count,
+                      uint64_t seed, int reps, float* ms, uint64_t* checksum) {
+    if (!rows || !queries || !ms || !checksum || count == 0 || reps < 1 || rows->n == 0 ||
+        queries->n == 0)
+        return setError(PANN_ERR_ARG, "invalid gather bench arguments");
+    uint32_t nq = 1;
+    while (nq * 2 <= queries->n && nq < (1u << 30)) nq *= 2;
+    int cus = 0;
+    int r = deviceCUs(&cus);
+    if (r != PANN_OK) return r;
+    unsigned long long* sum = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&sum, sizeof(*sum));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // one launch to warm up, then `reps` timed launches; the checksum is the last one's
+    for (int i = 0; e == hipSuccess && i <= reps; i++) {
+        if (i == 1) e = hipEventRecord(e0, nullptr);
+        if (e == hipSuccess) e = hipMemsetAsync(sum, 0, sizeof(*sum), nullptr);
+        if (e == hipSuccess)
+            e = launchGatherBench(rows->dev, rows->n, queries->dev, nq, count, seed, sum, 8, cus, nullptr);
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float total = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&total, e0, e1);
+    unsigned long long host = 0;
+    if (e == hipSuccess) e = hipMemcpy(&host, sum, sizeof(host), hipMemcpyDeviceToHost);
+    hipFree(sum);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "gather bench: %s", hipGetErrorString(e));
+    *ms = total / reps;
+    *checksum = host;
+    return PANN_OK;
+}
+
 // ---------------------------------------------------------------- vectors
 int pann_vectors_create(const int8_t* host_rows, uint32_t n, uint32_t dim, pann_vectors** out) {
     if (!out) return setError(PANN_ERR_ARG, "out is NULL");
@@ -260,9 +297,10 @@ int pann_graph_create(uint32_t n, uint32_t max_degree, const uint32_t* degrees,
     *out = nullptr;
     if (n == 0 || !degrees) return setError(PANN_ERR_ARG, "empty graph");
     uint64_t total = 0;
+    for (uint32_t i = 0; i < n; i++) total += degrees[i];
+    if (total > 0 && !edges) return setError(PANN_ERR_ARG, "edges is NULL");
     int r = fromStatus(checkGraph(n, max_degree, degrees, edges, &total));
     if (r != PANN_OK) return r;
-    if (total > 0 && !edges) return setError(PANN_ERR_ARG, "edges is NULL");
     if (max_degree == 0) max_degree = 1;
     if (max_degree > 256)
         return setError(PANN_ERR_CAPACITY, "max degree %u above the supported 256", max_degree);
@@ -389,13 +427,11 @@ int pann_hnsw_create(uint32_t n, const uint32_t* level, const uint32_t* node_poi
             h->nodeVec = base->dev;
             h->ownVec = false;
         } else {
-            // vectors in node order, copied device to device
+            // vectors in node order: one gather launch over the node -> point map
             e = hipMalloc(&h->nodeVec, (size_t)n * kVecBytes);
             h->ownVec = e == hipSuccess;
-            for (uint32_t i = 0; e == hipSuccess && i < n; i++)
-                e = hipMemcpyAsync(h->nodeVec + (size_t)i * kVecBytes,
-                                   base->dev + (size_t)node_point[i] * kVecBytes, kVecBytes,
-                                   hipMemcpyDeviceToDevice, nullptr);
+            if (e == hipSuccess)
+                e = launchGatherRows(h->nodeVec, base->dev, h->nodePoint, n, base->n, nullptr);
             if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         }
     }
@@ -536,7 +572,10 @@ int pann_searcher_launch(pann_searcher* s, const pann_vectors* base, const pann_
     if (!s || !base || !queries || !g) return setError(PANN_ERR_ARG, "NULL argument");
     if (s->kind != 0) return setError(PANN_ERR_ARG, "searcher was prepared for HNSW");
     if (base->n != g->n) return setError(PANN_ERR_ARG, "base has %u rows, graph %u nodes", base->n, g->n);
-    if (g->G != s->L.G) return setError(PANN_ERR_ARG, "graph differs from the prepared one");
+    // the prepared launch derives its addressing form and default visit limit from n and G
+    if (g->G != s->L.G || g->n != s->L.n)
+        return setError(PANN_ERR_ARG, "graph (%u nodes, degree %u) differs from the prepared one "
+                        "(%u, %u)", g->n, g->G, s->L.n, s->L.G);
     if (base->dim != queries->dim)
         return setError(PANN_ERR_ARG, "dim mismatch base %u queries %u", base->dim, queries->dim);
     if (queries->n < s->nq) return setError(PANN_ERR_ARG, "%u queries, searcher prepared for %u", queries->n, s->nq);
@@ -652,8 +691,17 @@ int pann_hnsw_searcher_launch(pann_searcher* s, const pann_vectors* queries, con
     if (queries->dim != h->dim)
         return setError(PANN_ERR_ARG, "dim mismatch index %u queries %u", h->dim, queries->dim);
     if (queries->n < s->nq) return setError(PANN_ERR_ARG, "%u queries, searcher prepared for %u", queries->n, s->nq);
+    // list width and LDS were sized for the prepared index's entry count
+    if (hnswListWidth(s->cfg.Ls, h->numEntries) != s->wr)
+        return setError(PANN_ERR_ARG, "index has %u entry points; the searcher was sized for "
+                        "lists of %d registers", h->numEntries, s->wr);
     if (s->nq == 0) return PANN_OK;
     HnswLaunch H = s->H;
+    // the top level and entry points come from the index of this launch
+    // (BuscadorHnsw::lanzar, cuda/busqueda_hnsw.cu:588-596)
+    H.top = (int32_t)h->top;
+    H.numEntries = (int32_t)h->numEntries;
+    for (int e = 0; e < 8; e++) H.entries[e] = e < (int)h->numEntries ? h->entries[e] : 0;
     H.vec = h->nodeVec;
     H.normsInRow = h->dim <= (uint32_t)kNormInRowMaxDim;
     H.adj0 = h->adj0;

@@ -262,6 +262,14 @@ hipError_t launchRowNorms(int8_t* rows, uint32_t n, int32_t* out, bool embed, hi
 // Per-section cycle counters of the flat search (profiling builds only).
 hipError_t profileCounters(uint64_t* out, int n, bool reset);
 hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s);
+// Random-gather microbenchmark over kVecBytes rows (see rows.hip); adds the sum of the
+// count dot products into *sum.
+hipError_t launchGatherBench(const int8_t* rows, uint32_t nrows, const int8_t* queries,
+                             uint32_t nq, uint64_t count, uint64_t seed,
+                             unsigned long long* sum, int blocksPerCU, int cus, hipStream_t s);
+// dst row i = src row idx[i] (kVecBytes rows); indices >= nsrc leave the row untouched.
+hipError_t launchGatherRows(int8_t* dst, const int8_t* src, const uint32_t* idx, uint32_t n,
+                            uint32_t nsrc, hipStream_t s);
 hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_t n,
                                const int8_t* queries, const int32_t* qnorm, uint32_t nq,
                                uint32_t k, bool normsInRow, uint32_t* outIds, int32_t* outDists,

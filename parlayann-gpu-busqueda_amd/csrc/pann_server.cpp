@@ -340,8 +340,10 @@ void pann_server_free(pann_server* s) {
 int pann_server_run_poisson(pann_server* s, const int8_t* rows, uint32_t nrows, double rate_qps,
                             double duration_s, uint32_t max_request, int32_t Ls_lo, int32_t Ls_hi,
                             const int32_t* ks, uint32_t nks, uint64_t seed,
-                            pann_server_stats* stats, uint32_t* record_ids, uint32_t* record_rows,
-                            uint32_t record_cap, uint32_t* recorded) {
+                            pann_server_stats* stats, uint32_t* record_ids, int32_t* record_dists,
+                            int32_t* record_flags, uint32_t* record_rows,
+                            pann_query_params* record_params, uint32_t record_cap,
+                            uint32_t* recorded) {
     if (!s || !rows || nrows == 0 || !(rate_qps > 0) || !(duration_s > 0) || max_request == 0 ||
         Ls_lo < 1 || Ls_hi < Ls_lo || !ks || nks == 0 || !stats)
         return setError(PANN_ERR_ARG, "invalid load generator arguments");
@@ -356,7 +358,7 @@ int pann_server_run_poisson(pann_server* s, const int8_t* rows, uint32_t nrows, 
     struct Sent {
         uint64_t id;
         std::vector<uint32_t> rowIdx;
-        std::vector<int32_t> k;
+        std::vector<pann_query_params> params;
     };
     std::vector<Sent> sent;
     std::mutex sentMu;
@@ -371,6 +373,7 @@ int pann_server_run_poisson(pann_server* s, const int8_t* rows, uint32_t nrows, 
     std::thread waiter([&] {
         size_t next = 0;
         std::vector<uint32_t> ids;
+        std::vector<int32_t> dists, flags;
         while (true) {
             Sent cur;
             {
@@ -380,20 +383,26 @@ int pann_server_run_poisson(pann_server* s, const int8_t* rows, uint32_t nrows, 
                 cur = sent[next++];
             }
             ids.resize(cur.rowIdx.size() * (size_t)K);
+            dists.resize(cur.rowIdx.size() * (size_t)K);
+            flags.resize(cur.rowIdx.size());
             double ms = 0;
-            if (pann_server_wait(s, cur.id, ids.data(), nullptr, nullptr, &ms) != PANN_OK) {
+            if (pann_server_wait(s, cur.id, ids.data(), dists.data(), flags.data(), &ms) != PANN_OK) {
                 err = PANN_ERR_HIP;
                 continue;
             }
             lat.push_back(ms);
             servedQueries += cur.rowIdx.size();
-            if (record_ids)
-                for (size_t i = 0; i < cur.rowIdx.size() && rec < record_cap; i++, rec++) {
-                    const uint32_t kk = (uint32_t)std::min<int32_t>(cur.k[i], 10);
-                    for (uint32_t j = 0; j < 10; j++)
-                        record_ids[(size_t)rec * 10 + j] = j < kk ? ids[i * K + j] : PANN_NO_NEIGHBOR;
-                    if (record_rows) record_rows[rec] = cur.rowIdx[i];
+            // the first min(k, 10) columns of each served query, in service order
+            for (size_t i = 0; i < cur.rowIdx.size() && rec < record_cap; i++, rec++) {
+                const uint32_t kk = (uint32_t)std::min<int32_t>(cur.params[i].k, 10);
+                for (uint32_t j = 0; j < 10; j++) {
+                    if (record_ids) record_ids[(size_t)rec * 10 + j] = j < kk ? ids[i * K + j] : PANN_NO_NEIGHBOR;
+                    if (record_dists) record_dists[(size_t)rec * 10 + j] = j < kk ? dists[i * K + j] : 0x7FFFFFFF;
                 }
+                if (record_flags) record_flags[rec] = flags[i];
+                if (record_rows) record_rows[rec] = cur.rowIdx[i];
+                if (record_params) record_params[rec] = cur.params[i];
+            }
         }
     });
 
@@ -416,7 +425,7 @@ int pann_server_run_poisson(pann_server* s, const int8_t* rows, uint32_t nrows, 
             const int32_t k = ks[rng() % nks];
             par[i] = pann_query_params{row, std::max(Ls_lo, std::min(Ls_hi, Ls)), k, 0, 0, 1.35};
             sd.rowIdx.push_back(row);
-            sd.k.push_back(k);
+            sd.params.push_back(par[i]);
         }
         std::this_thread::sleep_until(start + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(t)));
         if (pann_server_submit(s, buf.data(), par.data(), nq, &sd.id) != PANN_OK) {

@@ -88,6 +88,8 @@ PANN_SIGNATURES = {
     "pann_event_elapsed_ms": (C.c_int, [_P, _P, C.POINTER(C.c_float)]),
     "pann_event_destroy": (None, [_P]),
     "pann_copy_kernel": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "pann_gather_bench": (C.c_int, [_P, _P, _u64, _u64, C.c_int, C.POINTER(C.c_float),
+                                    C.POINTER(_u64)]),
     "pann_profile_counters": (C.c_int, [_P, C.c_int, C.c_int]),
     "pann_vectors_create": (C.c_int, [_P, _u32, _u32, _PP]),
     "pann_vectors_load": (C.c_int, [C.c_char_p, _PP]),
@@ -129,8 +131,8 @@ PANN_SIGNATURES = {
     "pann_server_stats_get": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "pann_server_free": (None, [_P]),
     "pann_server_run_poisson": (C.c_int, [_P, _P, _u32, C.c_double, C.c_double, _u32, _i32, _i32,
-                                          _P, _u32, _u64, C.POINTER(ServerStats), _P, _P, _u32,
-                                          C.POINTER(_u32)]),
+                                          _P, _u32, _u64, C.POINTER(ServerStats), _P, _P, _P,
+                                          _P, _P, _u32, C.POINTER(_u32)]),
 }
 
 TOOLS_SIGNATURES = {

@@ -376,6 +376,16 @@ def autotune(rows, params, max_Ls, max_k, base=None, graph=None, hnsw=None):
     return best, ms.value
 
 
+@dataclass
+class Recorded:
+    """Served queries recorded by the load generator, in service order."""
+    ids: np.ndarray     # [m, 10]: first min(k, 10) ids, NO_NEIGHBOR after
+    dists: np.ndarray   # [m, 10]
+    flags: np.ndarray   # [m]
+    rows: np.ndarray    # [m]: row of the query set (also the query's qid)
+    params: np.ndarray  # [m] PARAMS_DTYPE
+
+
 class Server:
     """Continuous-batching serving loop over a resident index (pann_server_*)."""
 
@@ -413,22 +423,30 @@ class Server:
         return {f: getattr(s, f) for f, _ in s._fields_}
 
     def run_poisson(self, rows, rate_qps, duration_s, max_request=4096, Ls_lo=10, Ls_hi=300,
-                    ks=(10, 100), seed=5, record_cap=0):
-        """Open-loop Poisson load; returns (stats dict, recorded ids [m, 10], recorded rows)."""
+                    ks=(10, 100), seed=5, record_cap=0, full_record=False):
+        """Open-loop Poisson load.  Returns (stats dict, recorded ids [m, 10], recorded rows),
+        or with full_record=True (stats, Recorded) holding ids, dists [m, 10], flags, rows
+        and the parameters of the first m <= record_cap served queries."""
         rows = np.ascontiguousarray(rows, dtype=np.int8)
         ks = np.ascontiguousarray(ks, dtype=np.int32)
         s = _lib.ServerStats()
-        rec_ids = np.empty((max(1, record_cap), 10), np.uint32)
-        rec_rows = np.empty(max(1, record_cap), np.uint32)
+        cap = max(1, record_cap)
+        rec = Recorded(np.empty((cap, 10), np.uint32), np.empty((cap, 10), np.int32),
+                       np.empty(cap, np.int32), np.empty(cap, np.uint32),
+                       np.zeros(cap, PARAMS_DTYPE))
         got = C.c_uint32()
+        on = (lambda a: _ptr(a) if record_cap else None)
         check(lib().pann_server_run_poisson(self.handle, _ptr(rows), len(rows), float(rate_qps),
                                             float(duration_s), max_request, Ls_lo, Ls_hi, _ptr(ks),
-                                            len(ks), seed, C.byref(s),
-                                            _ptr(rec_ids) if record_cap else None,
-                                            _ptr(rec_rows) if record_cap else None, record_cap,
-                                            C.byref(got)))
+                                            len(ks), seed, C.byref(s), on(rec.ids), on(rec.dists),
+                                            on(rec.flags), on(rec.rows), on(rec.params),
+                                            record_cap, C.byref(got)))
         st = {f: getattr(s, f) for f, _ in s._fields_}
-        return st, rec_ids[:got.value], rec_rows[:got.value]
+        m = got.value
+        rec = Recorded(rec.ids[:m], rec.dists[:m], rec.flags[:m], rec.rows[:m], rec.params[:m])
+        if full_record:
+            return st, rec
+        return st, rec.ids, rec.rows
 
     def free(self):
         if self.handle:
@@ -461,6 +479,15 @@ class Stream:
             self.free()
         except Exception:
             pass
+
+
+def gather_bench(rows, queries, count, seed=1, reps=3):
+    """Random-gather ceiling over the rows of `rows` (pann_gather_bench): returns (mean ms
+    per launch, checksum, gathered GB/s counting 128 bytes per distance)."""
+    ms, cs = C.c_float(), C.c_uint64()
+    check(lib().pann_gather_bench(rows.handle, queries.handle, int(count), int(seed), int(reps),
+                                  C.byref(ms), C.byref(cs)))
+    return ms.value, cs.value, 128.0 * count / (ms.value * 1e-3) / 1e9
 
 
 def bruteforce_knn(base, queries, k, nq=None, stream=None):

@@ -52,6 +52,11 @@ def test_argument_errors_are_status_codes():
     assert L.pann_searcher_prepare(None, 1, None, C.byref(out)) == _lib.PANN_ERR_ARG
     assert L.pann_searcher_outputs(None, None) == _lib.PANN_ERR_ARG
     assert L.pann_copy_kernel(None, None, 3, None) == _lib.PANN_ERR_ARG
+    # edges present by the degrees but a NULL edge pointer: refused before anything reads it
+    deg = (C.c_uint32 * 3)(2, 1, 0)
+    out = C.c_void_p()
+    assert L.pann_graph_create(3, 2, deg, None, C.byref(out)) == _lib.PANN_ERR_ARG
+    assert b"edges is NULL" in L.pann_last_error() and out.value is None
 
 
 def test_device_count_without_device_is_an_error_code():

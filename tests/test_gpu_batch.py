@@ -24,12 +24,16 @@ def _oracle_params(params):
 
 
 def _check_against_oracle(r, o, params, tag):
+    from pann import search
     for i in range(len(params)):
         k = int(params["k"][i])
         same = (np.array_equal(r.ids[i, :k], o.ids[i, :k]) and np.array_equal(r.dists[i, :k], o.dists[i, :k])
                 and r.comps[i] == o.comps[i] and r.flags[i] == o.flags[i] and r.hops[i] == o.hops[i])
         assert same, (f"{tag}: query {i} (Ls={params['Ls'][i]}, k={k}) gpu {r.ids[i, :k]} c{r.comps[i]} "
                       f"f{r.flags[i]} | oracle {o.ids[i, :k]} c{o.comps[i]} f{o.flags[i]}")
+        # columns past the query's k hold the missing-neighbour sentinel
+        assert (r.ids[i, k:] == search.NO_NEIGHBOR).all() and (r.dists[i, k:] == 0x7FFFFFFF).all(), (
+            f"{tag}: query {i} (k={k}) has values past its k")
 
 
 def test_flat_batch_matches_oracle_and_plain(gpu, small):
@@ -131,11 +135,11 @@ def test_batch_shapes_agree(gpu, small, wpb, bits, slots):
     other.launch()
     search.synchronize()
     got = other.download()
-    # only the first k columns of a query are written
-    for i, k in enumerate(params["k"]):
-        assert np.array_equal(got.ids[i, :k], want.ids[i, :k]), i
-        assert np.array_equal(got.dists[i, :k], want.dists[i, :k]), i
+    # every column is written: the first k from the frontier, the rest sentinels
+    assert np.array_equal(got.ids, want.ids) and np.array_equal(got.dists, want.dists)
     assert np.array_equal(got.comps, want.comps) and np.array_equal(got.flags, want.flags)
+    for i, k in enumerate(params["k"]):
+        assert (got.ids[i, k:] == search.NO_NEIGHBOR).all(), i
     ref.free(), other.free(), g.free(), b.free()
 
 

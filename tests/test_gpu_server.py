@@ -59,6 +59,8 @@ def test_mix_equals_alone_and_oracle(gpu, small):
         for i, k in enumerate(params["k"]):
             assert np.array_equal(got[0][i, :k], ids[i, :k]) and np.array_equal(got[1][i, :k], d[i, :k])
             assert np.array_equal(got[0][i, :k], o.ids[i, :k]) and np.array_equal(got[1][i, :k], o.dists[i, :k])
+            # columns past the query's k are the sentinel, not another micro-batch's values
+            assert (got[0][i, k:] == search.NO_NEIGHBOR).all() and (got[1][i, k:] == 0x7FFFFFFF).all()
         assert np.array_equal(got[2], f) and np.array_equal(f, o.flags)
     alone_srv.free(), g.free(), b.free()
 
