@@ -169,6 +169,22 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
  * device array of nq query ids (NULL: the row index, as the reference). */
 int pann_searcher_launch(pann_searcher* s, const pann_vectors* base, const pann_vectors* queries,
                          const pann_graph* g, const uint32_t* qids, void* stream);
+/* No counterpart in the reference.  pann_searcher_prepare + pann_searcher_launch for one
+ * configuration in a single call that allocates no device memory, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The caller owns the inputs and the device outputs: out_ids / out_dists
+ * nq x max(1, k), out_comps / out_flags / out_edges / out_hops nq each (those four may be
+ * NULL).  Long beams' per-wavefront scratch comes from a static device array of
+ * PANN_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends; a configuration
+ * whose one block needs more returns PANN_ERR_CAPACITY.  All calls on one GPU share that
+ * array: calls that overlap on the same GPU race and must be serialised by the caller.
+ * Asynchronous on `stream`; results equal pann_searcher_launch's. */
+#define PANN_NOALLOC_SCRATCH_BYTES (64ull * 1024 * 1024)
+int pann_search_flat_noalloc(const pann_vectors* base, const pann_graph* g,
+                             const pann_vectors* queries, const uint32_t* qids, uint32_t nq,
+                             const pann_search_config* cfg, uint32_t* out_ids, int32_t* out_dists,
+                             int32_t* out_comps, int32_t* out_flags, int32_t* out_edges,
+                             int32_t* out_hops, void* stream);
 /* BuscadorHnsw::preparar / lanzar (cuda/busqueda_hnsw.cu:551-630). */
 int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_search_config* cfg,
                                pann_searcher** out);

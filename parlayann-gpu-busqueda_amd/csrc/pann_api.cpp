@@ -475,48 +475,45 @@ void pann_hnsw_free(pann_hnsw* h) {
     delete h;
 }
 
-// ---------------------------------------------------------------- flat searcher
-int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_config* cfg,
-                          pann_searcher** out) {
-    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
-    *out = nullptr;
+}  // extern "C"
+
+namespace pann {
+
+// The launch of one flat search configuration over nq queries on graph g (the sizing of
+// BuscadorGPU::preparar, cuda/busqueda.cu:525-568): kernel form, LDS layout, wavefronts per
+// block, resident blocks, and the HBM scratch each wavefront needs (a seen filter above
+// kFilterBitsLds bits, a visited table from kVisitedHbmMinCap slots).  Scratch pointers
+// and outputs are left for the caller.
+int flatShape(const pann_graph* g, uint32_t nq, const pann_search_config* cfg, FlatShape* out) {
     if (!g) return setError(PANN_ERR_ARG, "graph is NULL");
     int r = checkConfig(cfg);
     if (r != PANN_OK) return r;
-    const int maxch = g->G <= 64 ? 1 : g->G <= 128 ? 2 : 4;
+    FlatShape sh;
+    sh.maxch = g->G <= 64 ? 1 : g->G <= 128 ? 2 : 4;
     const int Ls = cfg->Ls;
-    const int fbits = flatFilterBits(Ls);
+    sh.fbits = flatFilterBits(Ls);
     const uint32_t capV = flatVisitedCap(Ls);
-    const bool fast = maxch == 1 && Ls <= kFastLsMax;             // one frontier key per lane
-    const bool fast2 = maxch == 1 && !fast && Ls <= kFast2LsMax;  // two, filter in HBM
-    const bool ldsFilter = fbits <= kFilterBitsLds && !fast2;
-    const bool hbmVisited = !fast && !fast2 && capV >= kVisitedHbmMinCap;
-    const int fastKind = fast ? fastMode(g->n, g->G) : fast2 ? 8 + fastMode(g->n, g->G) : hbmVisited ? -1 : 0;
+    const bool fast = sh.maxch == 1 && Ls <= kFastLsMax;             // one frontier key per lane
+    const bool fast2 = sh.maxch == 1 && !fast && Ls <= kFast2LsMax;  // two, filter in HBM
+    sh.ldsFilter = sh.fbits <= kFilterBitsLds && !fast2;
+    sh.hbmVisited = !fast && !fast2 && capV >= kVisitedHbmMinCap;
+    const int fastKind = fast ? fastMode(g->n, g->G) : fast2 ? 8 + fastMode(g->n, g->G) : sh.hbmVisited ? -1 : 0;
     const FlatLds lay = fast ? flatFastLdsLayout(capV)
                       : fast2 ? flatFast2LdsLayout(capV)
-                              : flatLdsLayout(Ls, maxch, hbmVisited ? 0 : capV, ldsFilter ? fbits : 0);
+                              : flatLdsLayout(Ls, sh.maxch, sh.hbmVisited ? 0 : capV, sh.ldsFilter ? sh.fbits : 0);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "Ls=%d needs %u bytes of LDS per query", Ls, lay.total);
-
-    pann_searcher* s = new pann_searcher();
-    s->kind = 0;
-    s->cfg = *cfg;
-    s->nq = nq;
-    s->maxch = maxch;
     int bpc = 0, cus = 0;
     r = pickWavesPerBlock(lay.total, [&](int wpb, size_t lds, int* b) {
-            return beamFlatOccupancy(maxch, fastKind, wpb, lds, b); }, &s->wavesPerBlock, &bpc);
+            return beamFlatOccupancy(sh.maxch, fastKind, wpb, lds, b); }, &sh.wavesPerBlock, &bpc);
     if (r == PANN_OK) r = deviceCUs(&cus);
-    if (r != PANN_OK) {
-        delete s;
-        return r;
-    }
-    hipError_t he = hipSuccess;
-    const uint32_t want = (nq + s->wavesPerBlock - 1) / s->wavesPerBlock;
-    s->blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>(want, (uint32_t)std::max(1, bpc) * cus));
-    const uint32_t totalWaves = (uint32_t)s->blocks * s->wavesPerBlock;
+    if (r != PANN_OK) return r;
+    const uint32_t want = (nq + sh.wavesPerBlock - 1) / sh.wavesPerBlock;
+    sh.blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>(want, (uint32_t)std::max(1, bpc) * cus));
+    sh.scratchPerWave = (sh.ldsFilter ? 0 : ((size_t)4 << sh.fbits)) + (sh.hbmVisited ? (size_t)capV * 5 : 0);
 
-    FlatLaunch& L = s->L;
+    FlatLaunch& L = sh.L;
+    L = FlatLaunch{};
     L.n = g->n;
     L.G = g->G;
     L.nq = nq;
@@ -525,32 +522,69 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
     L.limit = cfg->visit_limit > 0 ? cfg->visit_limit : (int64_t)g->n;
     L.cut = cfg->cut;
     L.LsMax = Ls;
-    L.fbitsLds = ldsFilter ? fbits : 0;
-    L.gbitsMax = ldsFilter ? 0 : fbits;
+    L.fbitsLds = sh.ldsFilter ? sh.fbits : 0;
+    L.gbitsMax = sh.ldsFilter ? 0 : sh.fbits;
     L.capV = capV;
     L.ldsPerWave = lay.total;
     L.lds = lay;
     L.fast = fastKind;
-    L.totalWaves = totalWaves;
+    L.totalWaves = (uint32_t)sh.blocks * sh.wavesPerBlock;
     L.queue = nullptr;
-    if (!ldsFilter) {
-        he = hipMalloc(&s->gfilt, ((size_t)totalWaves << fbits) * 4);
+    L.outStride = (uint32_t)std::max(1, cfg->k);
+    *out = sh;
+    return PANN_OK;
+}
+
+// Points the launch at `totalWaves` wavefronts' scratch starting at `scratch`: the seen
+// filters, then the visited keys, then their counts.
+void flatBindScratch(FlatShape& sh, unsigned char* scratch) {
+    FlatLaunch& L = sh.L;
+    const size_t waves = L.totalWaves;
+    size_t off = 0;
+    L.gfilt = nullptr;
+    L.gvkey = nullptr;
+    L.gvcnt = nullptr;
+    if (!sh.ldsFilter) {
+        L.gfilt = reinterpret_cast<uint32_t*>(scratch);
+        off += (waves << sh.fbits) * 4;
+    }
+    if (sh.hbmVisited) {
+        L.gvkey = reinterpret_cast<uint32_t*>(scratch + off);
+        L.gvcnt = scratch + off + waves * L.capV * 4;
+    }
+}
+
+}  // namespace pann
+
+extern "C" {
+
+// ---------------------------------------------------------------- flat searcher
+int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_config* cfg,
+                          pann_searcher** out) {
+    if (!out) return setError(PANN_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    FlatShape sh;
+    int r = flatShape(g, nq, cfg, &sh);
+    if (r != PANN_OK) return r;
+    pann_searcher* s = new pann_searcher();
+    s->kind = 0;
+    s->cfg = *cfg;
+    s->nq = nq;
+    s->maxch = sh.maxch;
+    s->wavesPerBlock = sh.wavesPerBlock;
+    s->blocks = sh.blocks;
+    const size_t scratch = sh.scratchPerWave * sh.L.totalWaves;
+    if (scratch) {
+        const hipError_t he = hipMalloc(&s->gvis, scratch);
         if (he != hipSuccess) {
             delete s;
-            return setError(PANN_ERR_HIP, "allocating filter scratch: %s", hipGetErrorString(he));
+            return setError(PANN_ERR_HIP, "allocating %zu bytes of search scratch: %s", scratch,
+                            hipGetErrorString(he));
         }
     }
-    L.gfilt = s->gfilt;
-    if (hbmVisited) {
-        const size_t slots = (size_t)totalWaves * capV;
-        he = hipMalloc(&s->gvis, slots * 5);
-        if (he != hipSuccess) {
-            pann_searcher_free(s);
-            return setError(PANN_ERR_HIP, "allocating visited tables: %s", hipGetErrorString(he));
-        }
-        L.gvkey = s->gvis;
-        L.gvcnt = reinterpret_cast<unsigned char*>(s->gvis + slots);
-    }
+    flatBindScratch(sh, reinterpret_cast<unsigned char*>(s->gvis));
+    s->L = sh.L;
+    FlatLaunch& L = s->L;
     r = allocOutputs(s->out, nq, (uint32_t)std::max(1, cfg->k));
     if (r != PANN_OK) {
         pann_searcher_free(s);

@@ -55,6 +55,20 @@ namespace pann {
 
 int setError(int code, const char* fmt, ...);
 int deviceCUs(int* cus);
+
+// Launch shape of one flat search configuration (see flatShape in pann_api.cpp).
+struct FlatShape {
+    FlatLaunch L{};             // filled but for scratch pointers, inputs and outputs
+    int maxch = 1;              // 64-entry chunks of an adjacency row
+    int wavesPerBlock = 1;
+    int blocks = 1;             // resident blocks (L.totalWaves = blocks * wavesPerBlock)
+    int fbits = 10;             // seen filter bits
+    bool ldsFilter = true;      // the filter lives in LDS (else per-wavefront HBM scratch)
+    bool hbmVisited = false;    // the visited table lives in per-wavefront HBM scratch
+    size_t scratchPerWave = 0;  // HBM scratch bytes per wavefront
+};
+int flatShape(const pann_graph* g, uint32_t nq, const pann_search_config* cfg, FlatShape* out);
+void flatBindScratch(FlatShape& sh, unsigned char* scratch);
 int allocOutputs(pann_search_outputs& o, uint32_t nq, uint32_t stride);
 void freeOutputs(pann_search_outputs& o);
 

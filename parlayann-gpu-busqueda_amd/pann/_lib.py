@@ -107,6 +107,8 @@ PANN_SIGNATURES = {
     "pann_hnsw_free": (None, [_P]),
     "pann_searcher_prepare": (C.c_int, [_P, _u32, C.POINTER(SearchConfig), _PP]),
     "pann_searcher_launch": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "pann_search_flat_noalloc": (C.c_int, [_P, _P, _P, _P, _u32, C.POINTER(SearchConfig), _P, _P,
+                                           _P, _P, _P, _P, _P]),
     "pann_hnsw_searcher_prepare": (C.c_int, [_P, _u32, C.POINTER(SearchConfig), _PP]),
     "pann_hnsw_searcher_launch": (C.c_int, [_P, _P, _P, _P]),
     "pann_searcher_outputs": (C.c_int, [_P, C.POINTER(SearchOutputs)]),
