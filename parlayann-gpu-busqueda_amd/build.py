@@ -44,7 +44,13 @@ def run(cmd):
     return r
 
 
+# Per-file device flags: the brute-force GEMM keeps MFMA results in VGPRs (no AGPR reads
+# before each compare).
+FILE_FLAGS = {"bf_knn.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def compile_one(src, obj, hip, extra=()):
+    extra = list(extra) + (FILE_FLAGS.get(os.path.basename(src), []) if hip else [])
     if hip:
         cmd = [HIPCC] + HIP_FLAGS + list(extra) + ["-c", src, "-o", obj]
         if src.endswith(".cpp"):

@@ -26,6 +26,37 @@ def test_bruteforce_matches_oracle(gpu, n, nq, dim, k):
         assert (ids[:, n:] == search.NO_NEIGHBOR).all()
 
 
+@pytest.mark.parametrize("n,nq,dim,k", [(300_000, 300, 100, 100), (270_001, 129, 128, 16)])
+def test_bruteforce_sampled_threshold(gpu, n, nq, dim, k):
+    """Bases above 262,144 rows take their candidate threshold from a sample of the base."""
+    from oracle import oracle as O
+    from pann import datagen, search
+    base, queries = datagen.gmm_dataset(n, nq, dim=dim, ncomp=64, seeds=(n, n + 1, n + 2))
+    b, q = search.Vectors(base), search.Vectors(queries)
+    try:
+        ids, d = search.bruteforce_knn(b, q, k)
+    finally:
+        q.free(), b.free()
+    oi, od = O.bruteforce_knn(base, queries, k)
+    assert np.array_equal(d, od) and np.array_equal(ids, oi)
+
+
+@pytest.mark.parametrize("slots", ["16", "40"])
+def test_bruteforce_retry_path(gpu, monkeypatch, slots):
+    """With candidate lists too short for most queries, the one-pass kernel finishes them."""
+    from oracle import oracle as O
+    from pann import datagen, search
+    monkeypatch.setenv("PANN_BFKNN_CANDIDATES", slots)
+    base, queries = datagen.gmm_dataset(20000, 200, dim=100, ncomp=9, seeds=(5, 6, 7))
+    b, q = search.Vectors(base), search.Vectors(queries)
+    try:
+        ids, d = search.bruteforce_knn(b, q, 50)
+    finally:
+        q.free(), b.free()
+    oi, od = O.bruteforce_knn(base, queries, 50)
+    assert np.array_equal(d, od) and np.array_equal(ids, oi)
+
+
 def test_bruteforce_duplicate_points_tie_by_id(gpu):
     """Many equal distances: order must be by id within a distance."""
     from oracle import oracle as O
