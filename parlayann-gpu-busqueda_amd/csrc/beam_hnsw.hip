@@ -431,12 +431,13 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
 }
 
 // On rows of at most 64 entries, two-register lists are held to 128 VGPRs (four wavefronts
-// per SIMD; a few spills) and five-register lists to 168 (three): the hop is a chain of
-// dependent memory round trips, and ef=100 / ef=256 on the 1M HNSW run 1.17x / 1.3x faster
-// than with the registers the compiler would take.
+// per SIMD; a few spills) and five- to seven-register lists to 168 (three): the hop is a
+// chain of dependent memory round trips, and ef=100 / ef=256 / ef=350 on the 1M HNSW run
+// 1.17x / 1.3x / 1.2x faster than with the registers the compiler would take.  Eight-
+// register lists spill too much at 168 (ef=500: 52.3 ms held, 47.7 ms not).
 template <int WR, int MAXCH, bool IG>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(MAXCH == 1 && WR == 2 ? 4 : MAXCH == 1 && WR == 5 ? 3 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(MAXCH == 1 && WR == 2 ? 4 : MAXCH == 1 && WR >= 5 && WR <= 7 ? 3 : 1, 8)))
 void beamHnswKernel(HnswLaunch L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = laneId();
@@ -517,6 +518,12 @@ hipError_t occT(bool inwHbm, int wpb, size_t lds, int* bpc) {
         case 51: return FN<5, 1>(__VA_ARGS__);                                \
         case 52: return FN<5, 2>(__VA_ARGS__);                                \
         case 54: return FN<5, 4>(__VA_ARGS__);                                \
+        case 61: return FN<6, 1>(__VA_ARGS__);                                \
+        case 62: return FN<6, 2>(__VA_ARGS__);                                \
+        case 64: return FN<6, 4>(__VA_ARGS__);                                \
+        case 71: return FN<7, 1>(__VA_ARGS__);                                \
+        case 72: return FN<7, 2>(__VA_ARGS__);                                \
+        case 74: return FN<7, 4>(__VA_ARGS__);                                \
         case 81: return FN<8, 1>(__VA_ARGS__);                                \
         case 82: return FN<8, 2>(__VA_ARGS__);                                \
         case 84: return FN<8, 4>(__VA_ARGS__);                                \

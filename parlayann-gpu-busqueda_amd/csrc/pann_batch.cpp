@@ -2,7 +2,7 @@
 // of queries that each carry their own qid, Ls, k, visit limit and cut; the launch can be
 // captured into a HIP graph and replayed; an autotuner picks the launch shape per index.
 // A batch is split by Ls class -- flat: <= 64, <= 128, the rest; HNSW: the widths of its
-// register-resident lists, Ls <= 55, <= 119, <= 183, <= 247, <= 311, the rest -- and each class drains in a
+// register-resident lists, Ls <= 55, <= 119, <= 183, <= 247, <= 311, <= 375, <= 439, the rest -- and each class drains in a
 // launch of its own, sized for the longest beam of that class, so short beams run at the
 // occupancy and list width of their own class rather than max_Ls's.  The class launches
 // run side by side (the narrower classes on the batch's own streams, forked from and
@@ -359,9 +359,9 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
     b->dim = h->dim;
     const uint32_t G = std::max(h->G0, h->GA);
     b->maxch = G <= 64 ? 1 : G <= 128 ? 2 : 4;
-    // classes by list width: Ls + 9 within 64, 128, 192, 256, 320 entries
+    // classes by list width: Ls + 9 within 64, 128, ..., 448 entries
     std::vector<int> his;
-    for (int hi : {55, 119, 183, 247, 311})
+    for (int hi : {55, 119, 183, 247, 311, 375, 439})
         if (hi < LsMax) his.push_back(hi);
     his.push_back(LsMax);
     b->parts.resize(his.size());

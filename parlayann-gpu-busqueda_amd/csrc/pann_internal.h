@@ -178,7 +178,7 @@ __host__ __device__ inline uint32_t hnswInwCap(int Ls) { return 1u << ceilLog2((
 inline int hnswListWidth(int Ls, uint32_t numEntries) {
     const int need = (Ls > (int)numEntries ? Ls : (int)numEntries) + 9;
     return need <= 64 ? 1 : need <= 128 ? 2 : need <= 192 ? 3 : need <= 256 ? 4 : need <= 320 ? 5
-         : need <= 512 ? 8 : 0;
+         : need <= 384 ? 6 : need <= 448 ? 7 : need <= 512 ? 8 : 0;
 }
 
 // Launches whose is_inw set has at least this many slots (ef > 112) keep layer 0's in HBM

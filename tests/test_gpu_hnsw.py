@@ -46,7 +46,8 @@ def built():
 
 
 @pytest.mark.parametrize("Ls,k", [(10, 10), (20, 10), (40, 10), (80, 10), (130, 10), (250, 10),
-                                  (1, 1), (5, 10), (64, 100), (300, 50)])
+                                  (1, 1), (5, 10), (64, 100), (300, 50), (350, 10), (430, 100),
+                                  (500, 10)])
 def test_built_hnsw_parity(gpu, built, Ls, k):
     base, queries, h = built
     _assert_same(_run(base, h, queries, Ls, k), _oracle(base, h, queries, Ls, k), k, f"Ls={Ls} k={k}")
