@@ -359,6 +359,20 @@ def batched_vs_plain(b, g, rows, params, steps, tune_sample=8192):
         s.launch(b, qv, g, qid, st.ptr)
     st.synchronize()
     plain_ms = (time.perf_counter() - t0) * 1e3
+    # the same launches spread over four streams, longest first: launch serialisation taken
+    # out of the plain form (each searcher owns its scratch, so they may run side by side)
+    streams = [search.Stream() for _ in range(4)]
+    order = sorted(range(len(plain)), key=lambda i: -int(keys[plain[i][0][0]][0]) * len(plain[i][0]))
+    _lib.check(L.pann_device_synchronize())
+    t0 = time.perf_counter()
+    for j, i in enumerate(order):
+        members, qv, qid, s = plain[i]
+        s.launch(b, qv, g, qid, streams[j % 4].ptr)
+    for x in streams:
+        x.synchronize()
+    plain4_ms = (time.perf_counter() - t0) * 1e3
+    for x in streams:
+        x.free()
     mismatches = 0
     for members, qv, qid, s in plain:
         r = s.download()
@@ -379,6 +393,9 @@ def batched_vs_plain(b, g, rows, params, steps, tune_sample=8192):
         "ratio": round(plain_ms / batched_ms, 2),
         "batched_ms": round(batched_ms, 3), "batched_kernel_ms": round(batched_kernel_ms, 3),
         "plain_ms": round(plain_ms, 3),
+        "qps_plain_4_streams": round(nq / (plain4_ms / 1e3), 1),
+        "ratio_vs_plain_4_streams": round(plain4_ms / batched_ms, 2),
+        "plain_4_streams_ms": round(plain4_ms, 3),
         "autotuned": {"waves_per_block": cfg.waves_per_block, "lds_filter_bits": cfg.lds_filter_bits,
                       "visited_slots": cfg.visited_slots, "tune_ms": round(tune_ms, 3)},
         "queries_differing_from_plain": mismatches, "overflow_or_config_flags": flags,
@@ -468,7 +485,7 @@ def sweep_graphs(args, base, queries, b, q, gt_ids, gt_df, k=10):
         build_s = time.time() - t
         idx = search.HnswIndex(host, b) if kind == "hnsw" else search.FlatGraph(host)
         log(f"[sweep] {kind}: built in {build_s:.1f}s")
-        for Ls in LS_SWEEP:
+        for Ls in ([int(x) for x in args.sweep_ls.split(",")] if args.sweep_ls else LS_SWEEP):
             cfg = search.SearchConfig(Ls=Ls, k=k)
             if kind == "hnsw":
                 s = search.HnswSearcher(idx, len(queries), cfg)
@@ -482,17 +499,18 @@ def sweep_graphs(args, base, queries, b, q, gt_ids, gt_df, k=10):
             rec = recall.recall_at_k(gt_ids, gt_df, r.ids, k)
             ms = min(event_ms(launch) for _ in range(3))
             s.free()
-            params = O.make_params(sample, Ls, k)
-            run = (lambda: O.search_hnsw(base, host, qs, params, nthreads=cores)) if kind == "hnsw" \
-                else (lambda: O.beam_flat(base, host, qs, params, nthreads=cores))
-            run()
             cpu = 0.0
-            for _ in range(2):
-                t = time.perf_counter()
+            if sample:
+                params = O.make_params(sample, Ls, k)
+                run = (lambda: O.search_hnsw(base, host, qs, params, nthreads=cores)) if kind == "hnsw" \
+                    else (lambda: O.beam_flat(base, host, qs, params, nthreads=cores))
                 run()
-                cpu = max(cpu, sample / (time.perf_counter() - t))
+                for _ in range(2):
+                    t = time.perf_counter()
+                    run()
+                    cpu = max(cpu, sample / (time.perf_counter() - t))
             row = {"graph": kind, "Ls": Ls, "recall": round(rec, 5), "gpu_kernel_ms": round(ms, 4),
-                   "gpu_qps": round(len(queries) / ms * 1e3, 1), "cpu_qps": round(cpu, 1),
+                   "gpu_qps": round(len(queries) / ms * 1e3, 1), "cpu_qps": round(cpu, 1) if sample else None,
                    "comps": round(float(r.comps.mean()), 1), "hops": round(float(r.hops.mean()), 1),
                    "flagged": int(np.count_nonzero(r.flags & ~search.FLAG_SHORT)),
                    "alg_gbs": round(float(100 * r.comps.astype(np.float64).sum()
@@ -582,7 +600,8 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="instead of the headline line: the 18-Ls sweep over the four graph families")
     ap.add_argument("--sweep-graphs", default="vamana,hcnng,pynndescent,hnsw")
-    ap.add_argument("--sweep-cpu-sample", type=int, default=2000)
+    ap.add_argument("--sweep-cpu-sample", type=int, default=2000, help="0: no CPU leg")
+    ap.add_argument("--sweep-ls", default="", help="comma-separated Ls values (default: the 18)")
     ap.add_argument("--hnsw-large", action="store_true",
                     help="instead of the headline line: HNSW at 10M x 128 (ef 16/64/256)")
     ap.add_argument("--w4-n", type=int, default=10_000_000)
@@ -593,10 +612,10 @@ def main():
                     help="time the batched form in its default shape instead of the autotuned one")
     ap.add_argument("--mixed-nq", type=int, default=65_536)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serving-rates", default="200000,1000000,2000000")
+    ap.add_argument("--serving-rates", default="50000,200000,500000,1000000,2000000")
     ap.add_argument("--serving-seconds", type=float, default=2.0)
     ap.add_argument("--no-serving-hnsw", action="store_true")
-    ap.add_argument("--serving-hnsw-rates", default="200000,1000000")
+    ap.add_argument("--serving-hnsw-rates", default="50000,200000,500000,1000000,2000000")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run the copy kernel over this many bytes (counter calibration)")
@@ -714,7 +733,8 @@ def main():
         rows, params = mixed_workload(queries, args.mixed_nq)
         mixed = batched_vs_plain(b, g, rows, params, max(3, args.steps // 2))
         log(f"[bench] mixed: batched {mixed['qps_batched']:,.0f} QPS vs plain {mixed['qps_plain']:,.0f} "
-            f"QPS ({mixed['ratio']}x over {mixed['distinct_configs']} configs), "
+            f"QPS ({mixed['ratio']}x over {mixed['distinct_configs']} configs; "
+            f"{mixed['ratio_vs_plain_4_streams']}x over them on 4 streams), "
             f"{mixed['queries_differing_from_plain']} queries differ")
 
     serve = None
