@@ -29,6 +29,8 @@ import numpy as np  # noqa: E402
 
 METRIC = ("QPS @ recall@10=0.90, 1M×100 int8; batched vs plain on 1 GPU; p99 ms")
 LS_SWEEP = [10, 12, 15, 20, 25, 30, 35, 40, 50, 60, 75, 100, 125, 150, 200, 300, 400, 500]
+# graph families under the names the reference's result files use
+REPORT_NAMES = {"vamana": "ParDiskANN", "hcnng": "ParHCNNG", "pynndescent": "ParPyNNDescent", "hnsw": "ParHNSW"}
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_SOURCES = ("beam_flat.hip", "device_common.h", "pann_internal.h")
 
@@ -94,7 +96,7 @@ def time_launches(searcher, b, q, g, steps, warmup):
     return wall, tot / steps
 
 
-def end_to_end(searcher, b, q, g, queries, reps=5):
+def end_to_end(searcher, b, q, g, queries, reps=5, launch=None):
     """The reference's end-to-end protocol (cuda/comparacion1.cu:67-75, 96-109): query rows
     from pinned host memory to the device (pann_vectors_copy_async, which also refreshes the
     norms), the launch, ids and distances back to pinned host memory, stream synchronize;
@@ -122,7 +124,10 @@ def end_to_end(searcher, b, q, g, queries, reps=5):
 
     def run():
         _lib.check(L.pann_vectors_copy_async(q.handle, rows_p, nq, st.ptr))
-        searcher.launch(b, q, g, stream=st.ptr)
+        if launch is None:
+            searcher.launch(b, q, g, stream=st.ptr)
+        else:
+            launch(st.ptr)
         _lib.check(L.pann_searcher_download(searcher.handle, ids_p, dists_p, None, None, None, None,
                                             st.ptr))
         _lib.check(L.pann_synchronize(st.ptr))
@@ -463,12 +468,12 @@ def sweep_graphs(args, base, queries, b, q, gt_ids, gt_df, k=10):
     events, plain launch over all queries), recall@10, distance computations per query,
     and the oracle's QPS on all host cores over a bounded query sample."""
     from oracle import oracle as O
-    from pann import builders, recall, search
+    from pann import builders, recall, report, search
     cores = host_cores()
     nthreads = min(16, cores)
     sample = min(args.sweep_cpu_sample, len(queries))
     qs = queries[:sample]
-    rows = []
+    rows, table, validation = [], [], []
     for kind in args.sweep_graphs.split(","):
         t = time.time()
         if kind == "vamana":
@@ -498,19 +503,35 @@ def sweep_graphs(args, base, queries, b, q, gt_ids, gt_df, k=10):
             r = s.download()
             rec = recall.recall_at_k(gt_ids, gt_df, r.ids, k)
             ms = min(event_ms(launch) for _ in range(3))
+            e2e = None
+            if args.report:
+                e2e_launch = (lambda st: s.launch(q, idx, stream=st)) if kind == "hnsw" else None
+                e2e, _ = end_to_end(s, b, q, idx, queries, reps=3, launch=e2e_launch)
             s.free()
             cpu = 0.0
             if sample:
                 params = O.make_params(sample, Ls, k)
                 run = (lambda: O.search_hnsw(base, host, qs, params, nthreads=cores)) if kind == "hnsw" \
                     else (lambda: O.beam_flat(base, host, qs, params, nthreads=cores))
-                run()
+                ref = run()
                 for _ in range(2):
                     t = time.perf_counter()
                     run()
                     cpu = max(cpu, sample / (time.perf_counter() - t))
+                if args.report:
+                    v = report.validate(r.ids[:sample, :k], ref.ids[:, :k], base, qs)
+                    validation.append({"graph": REPORT_NAMES[kind], "Ls": Ls, "visit": -1, **v})
+                    table.append({"graph": REPORT_NAMES[kind], "Ls": Ls, "visit": -1,
+                                  "recall_cpu": recall.recall_at_k(gt_ids[:sample], gt_df[:sample], ref.ids, k),
+                                  "recall_gpu": rec,
+                                  "recall_gpu_sample": recall.recall_at_k(gt_ids[:sample], gt_df[:sample],
+                                                                          r.ids[:sample], k),
+                                  "qps_cpu": cpu, "qps_gpu_total": e2e["e2e_qps"],
+                                  "qps_gpu_kernel": len(queries) / ms * 1e3,
+                                  "deterministic": e2e["deterministic"]})
             row = {"graph": kind, "Ls": Ls, "recall": round(rec, 5), "gpu_kernel_ms": round(ms, 4),
                    "gpu_qps": round(len(queries) / ms * 1e3, 1), "cpu_qps": round(cpu, 1) if sample else None,
+                   "e2e_qps": e2e["e2e_qps"] if e2e else None,
                    "comps": round(float(r.comps.mean()), 1), "hops": round(float(r.hops.mean()), 1),
                    "flagged": int(np.count_nonzero(r.flags & ~search.FLAG_SHORT)),
                    "alg_gbs": round(float(100 * r.comps.astype(np.float64).sum()
@@ -520,6 +541,14 @@ def sweep_graphs(args, base, queries, b, q, gt_ids, gt_df, k=10):
             log(f"[sweep] {kind} Ls={Ls}: recall {rec:.5f}, {row['gpu_qps']:,.0f} QPS GPU "
                 f"({ms:.3f} ms), {cpu:,.0f} QPS CPU, comps {row['comps']}")
         idx.free()
+    if args.report:
+        notes = [f"{len(base):,} x {base.shape[1]} int8 base, {len(queries):,} queries, k = {k}.",
+                 f"CPU: the oracle on {cores} host threads over the first {sample:,} queries, best of 2 "
+                 "passes; recall CPU and the validation are over that sample, recall GPU over all queries.",
+                 "GPU kernel: HIP events, best of 3 launches, queries resident.  GPU total: query rows "
+                 "host to device + search + ids and distances back, best of 3 (steady clock)."]
+        files = report.write_report(args.report, table, validation, notes=notes)
+        log(f"[sweep] report: {', '.join(files)} in {args.report}")
     return {"sweep": rows, "cpu_cores": cores, "cpu_sample_queries": sample,
             "n": len(base), "nq": len(queries)}
 
@@ -602,6 +631,9 @@ def main():
     ap.add_argument("--sweep-graphs", default="vamana,hcnng,pynndescent,hnsw")
     ap.add_argument("--sweep-cpu-sample", type=int, default=2000, help="0: no CPU leg")
     ap.add_argument("--sweep-ls", default="", help="comma-separated Ls values (default: the 18)")
+    ap.add_argument("--report", default="",
+                    help="with --sweep: write the validation / recall-QPS report to this folder "
+                         "(adds an end-to-end leg per configuration)")
     ap.add_argument("--hnsw-large", action="store_true",
                     help="instead of the headline line: HNSW at 10M x 128 (ef 16/64/256)")
     ap.add_argument("--w4-n", type=int, default=10_000_000)

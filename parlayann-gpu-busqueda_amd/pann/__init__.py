@@ -6,6 +6,7 @@ Modules:
     datagen   synthetic Gaussian-mixture int8 data
     builders  CPU graph builders (Vamana, HNSW, HCNNG, PyNNDescent)
     recall    tie-aware recall@k
+    report    neighbour-by-neighbour validation and recall-QPS tables (python -m pann.report)
 """
 from . import datagen, formats, recall  # noqa: F401
 from ._lib import PannError  # noqa: F401
