@@ -311,6 +311,21 @@ int pann_server_run_poisson(pann_server* s, const int8_t* rows, uint32_t nrows, 
 int pann_bruteforce_knn(const pann_vectors* base, const pann_vectors* queries, uint32_t nq,
                         uint32_t k, uint32_t* ids_dev, int32_t* dists_dev, void* stream);
 
+/* ---- graph construction ---- */
+/* Vamana (DiskANN) graph over the rows of `base`, built on the device: the batch insertion
+ * of ParlayANN's Vamana (prefix-doubling batches up to 2% of n, per-point beam search of
+ * width L from node 0 with an exact visited set, robust prune with alpha to R, reverse
+ * edges grouped by target), which the reference names as its GPU build to add
+ * (README.md:346-349) and otherwise loads prebuilt (leerGrafo, cuda/lectores.cpp:97-144).
+ * two_pass runs the insertion once with alpha = 1 first.  The result equals
+ * pann_build_vamana (include/pann_tools.h) for the same arguments, bit for bit:
+ * out_degrees (n) and out_edges (n x R, PANN_NO_NEIGHBOR padded) are host arrays.
+ * R <= 128, L <= 256; PANN_ERR_CAPACITY if a search expands more than 1024 nodes or a
+ * prune meets more than 2048 candidates.  Blocking. */
+int pann_build_vamana_gpu(const pann_vectors* base, uint32_t R, uint32_t L, float alpha,
+                          int two_pass, uint64_t seed, uint32_t* out_degrees,
+                          uint32_t* out_edges, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

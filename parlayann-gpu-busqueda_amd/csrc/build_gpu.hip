@@ -1,0 +1,659 @@
+// build_gpu.hip -- Vamana (DiskANN) graph construction on gfx950.
+//
+// The algorithm is the host builder's (build_vamana.cpp), which follows ParlayANN's
+// published batch insertion (README.md:346-349 names it as the GPU build to add):
+// points in a seeded random order, batches doubling from 1 to 2% of n; per batch
+//   1. buildSearchKernel: each point runs a greedy beam search of width L from node 0
+//      on the pre-batch graph, with an exact visited set, and records the nodes it
+//      expanded with their distances;
+//   2. buildPruneKernel: each point robust-prunes (alpha) its expanded nodes plus its
+//      current row to at most R out-neighbours and writes its own row, emitting
+//      (target << 32 | source) for every new edge;
+//   3. the edges are radix-sorted (rocPRIM), so each target's new in-edges form one
+//      run in source order, and buildReverseKernel gives each run to one wavefront,
+//      which appends the absent sources to the target's row or, past R, robust-prunes
+//      the row together with them.
+// Every phase writes only rows it owns and reads the graph as the previous phase left
+// it, so the result is the host builder's graph, bit for bit.
+//
+// Search state of one wavefront: the frontier is the best L (by (distance, id)) of all
+// nodes marked so far, as 64-bit keys in C registers (element e = c*64 + lane) plus an
+// expanded bitmask per register.  A hop expands the first unexpanded entry, marks its
+// unmarked neighbours in a per-wavefront bitmap over all n ids (one atomic OR each,
+// which also answers "was it marked"), computes their distances (8 lanes per row,
+// v_dot4), sorts them and merges them in by rank.  The host search inserts neighbours
+// one at a time and drops a neighbour no better than a full frontier's last entry; the
+// result is the same top-L set, and the next node it expands is the same first
+// unexpanded entry.  At the end the wavefront clears exactly the bitmap words it set
+// (the start node and the rows of the expanded nodes), so bitmaps never need a sweep.
+//
+// Prunes run in two tiers: four wavefronts per block with room for 512 candidates
+// each, and the rare wider prunes (up to 2048 candidates) deferred to a second launch
+// with one wavefront per block.
+#include "device_common.h"
+#include "tools_common.h"
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <vector>
+
+namespace pann {
+namespace {
+
+using namespace dev;
+
+constexpr int kSearchWaves = 4;            // wavefronts per search block
+constexpr uint32_t kVisitCap = 1024;       // expanded nodes recorded per search
+constexpr uint32_t kSmallCap = 512;        // candidates of a prune in the common tier
+constexpr uint32_t kSortCap = 2048;        // candidates of a prune in the deferred tier
+constexpr int kPruneWaves = 4;             // wavefronts per common-tier prune block
+constexpr uint32_t kErrVisit = 1, kErrPrune = 2;
+constexpr size_t kMarkBudget = 16ull << 30; // bytes of search bitmaps at most
+
+__device__ __forceinline__ uint64_t keyOf(int d, uint32_t id) {
+    return ((uint64_t)(uint32_t)d << 32) | id;
+}
+
+struct BuildArgs {
+    const int8_t* vec;      // n rows, kVecBytes pitch, zero padded
+    int32_t normsInRow;     // rows hold |p|^2 in bytes 124..127
+    uint32_t n, R, L;
+    uint32_t* adj;          // n x R, kEmpty padded
+    uint32_t* deg;          // n
+    const uint32_t* batch;  // the batch's points
+    uint32_t size;
+    uint64_t* visited;      // size x kVisitCap keys (distance << 32 | id)
+    uint32_t* nvisited;     // size; kVisitCap + 1 marks an overflow
+    uint32_t* marks;        // per search wavefront: markWords bitmap words
+    uint32_t markWords;
+    uint32_t searchWaves;   // wavefronts of the search launch (the stride over the batch)
+    uint64_t* pairs;        // size x R new edges (target << 32 | source), ~0 unused
+    const uint64_t* sorted; // the pairs, sorted
+    uint32_t npairs;
+    const uint32_t* heads;  // first index of each target's run in sorted
+    const uint32_t* nheads;
+    uint32_t* deferred;     // items (batch or head indices) left for the wide tier
+    uint32_t* ndeferred;
+    uint32_t* err;
+};
+
+// ---- the search ----
+
+// Number of keys of the sorted register `ck` (64 lanes, ~0 padded) below x.
+__device__ __forceinline__ uint32_t countBelow64(uint64_t ck, uint64_t x) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+        const uint64_t probe = shfl64(ck, (int)(pos + s - 1));
+        if (probe < x) pos += s;
+    }
+    if (pos == 63 && readlane64(ck, 63) < x) pos = 64;
+    return pos;
+}
+
+// Number of the F frontier keys below y (frontier sorted over C registers).
+template <int C>
+__device__ __forceinline__ uint32_t countBelowFrontier(const uint64_t (&fk)[C], uint32_t F, uint64_t y) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (int s = 64 * C; s >= 1; s >>= 1) {
+        const uint32_t idx = pos + s - 1;   // probe index; lanes differ
+        uint64_t probe = ~0ull;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const uint64_t v = shfl64(fk[c], (int)(idx & 63));
+            if ((idx >> 6) == (uint32_t)c) probe = v;
+        }
+        if (pos + s <= F && probe < y) pos += s;
+    }
+    return pos;
+}
+
+// Merges the candidate keys ck (unsorted, ~0 padded, none already in the frontier) into
+// the frontier, keeping its best L; expanded flags travel with their entries.
+template <int C>
+__device__ void mergeFrontier(uint64_t (&fk)[C], uint64_t (&ex)[C], uint32_t& F, uint64_t ck,
+                              uint32_t L, uint64_t* mkey, unsigned char* mflag, int lane) {
+    uint64_t s1[1] = {ck};
+    bitonicSort<1>(s1, lane);
+    ck = s1[0];
+    const uint32_t nc = (uint32_t)__popcll(ballot(ck != ~0ull));
+    if (nc == 0) return;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const uint32_t e = c * 64 + lane;
+        const uint32_t np = e + countBelow64(ck, fk[c]);
+        if (e < F && np < L) {
+            mkey[np] = fk[c];
+            mflag[np] = (unsigned char)((ex[c] >> lane) & 1);
+        }
+    }
+    {
+        const uint32_t np = (uint32_t)lane + countBelowFrontier<C>(fk, F, ck);
+        if ((uint32_t)lane < nc && np < L) {
+            mkey[np] = ck;
+            mflag[np] = 0;
+        }
+    }
+    F = min(L, F + nc);
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const uint32_t e = c * 64 + lane;
+        fk[c] = e < F ? mkey[e] : ~0ull;
+        const bool fl = e < F && mflag[e] != 0;
+        ex[c] = ballot(fl);
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int C>
+__device__ void searchOne(const BuildArgs& a, uint32_t i, uint32_t* marks, uint64_t* mkey,
+                          unsigned char* mflag, uint32_t* pend, int lane) {
+    uint64_t* vis = a.visited + (size_t)i * kVisitCap;
+    const uint32_t p = a.batch[i];
+    const int sub = lane & 7, grp = lane >> 3;
+    const int normMask = (a.normsInRow && sub == 7) ? -1 : 0;
+    int4 qv = *reinterpret_cast<const int4*>(a.vec + (size_t)p * kVecBytes + sub * 16);
+    qv.w &= ~normMask;
+    const int qq = groupSum8(sliceNorm(qv));
+
+    uint64_t fk[C], ex[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        fk[c] = ~0ull;
+        ex[c] = 0;
+    }
+    // the start node 0
+    if (lane == 0) atomicOr(&marks[0], 1u);
+    {
+        const int d0 = readlane(groupDistance(a.vec, 0, sub, qv, qq, normMask), 0);
+        if (lane == 0) fk[0] = keyOf(d0, 0);
+    }
+    uint32_t F = 1, nv = 0;
+    bool overflow = false;
+    while (true) {
+        int e = -1;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const uint64_t m = ballot((uint32_t)(c * 64 + lane) < F) & ~ex[c];
+            if (e < 0 && m) e = c * 64 + __ffsll((unsigned long long)m) - 1;
+        }
+        if (e < 0) break;
+        if (nv >= kVisitCap) {
+            overflow = true;
+            break;
+        }
+        uint64_t cur = 0;
+#pragma unroll
+        for (int c = 0; c < C; c++)
+            if (c == (e >> 6)) {
+                cur = readlane64(fk[c], e & 63);
+                ex[c] |= 1ull << (e & 63);
+            }
+        if (lane == 0) vis[nv] = cur;
+        nv++;
+        const uint32_t u = (uint32_t)cur;
+        const uint32_t du = a.deg[u];
+        const uint32_t* row = a.adj + (size_t)u * a.R;
+        uint32_t np = 0;
+        for (uint32_t ch = 0; ch * 64 < du; ch++) {
+            const uint32_t j = ch * 64 + lane;
+            const uint32_t x = j < du ? row[j] : kEmpty;
+            bool fresh = false;
+            if (x != kEmpty) {
+                const uint32_t bit = 1u << (x & 31);
+                fresh = (atomicOr(&marks[x >> 5], bit) & bit) == 0;
+            }
+            const uint64_t fm = ballot(fresh);
+            if (fresh) pend[np + mbcnt(fm)] = x;
+            np += (uint32_t)__popcll(fm);
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t b0 = 0; b0 < np; b0 += 64) {
+            const uint32_t nb = min(64u, np - b0);
+            int4 v[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                v[r] = make_int4(0, 0, 0, 0);
+                const uint32_t t = (uint32_t)r * 8 + grp;
+                if (t < nb)
+                    v[r] = *reinterpret_cast<const int4*>(a.vec + (size_t)pend[b0 + t] * kVecBytes + sub * 16);
+            }
+            int part8[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                v[r].w &= ~normMask;
+                part8[r] = slicePartial(v[r], qv);
+            }
+            const int tot = transposeReduce8(part8, lane);
+            const int mine = __shfl(tot, (lane & 7) * 8 + (lane >> 3), 64);
+            const uint64_t ck = (uint32_t)lane < nb ? keyOf(qq + mine, pend[b0 + lane]) : ~0ull;
+            mergeFrontier<C>(fk, ex, F, ck, a.L, mkey, mflag, lane);
+        }
+    }
+    // clear the bitmap words this search set: node 0's and those of the expanded rows,
+    // whose ids are read back past the vector L1 (it does not see this wave's stores)
+    if (lane == 0) {
+        a.nvisited[i] = overflow ? kVisitCap + 1 : nv;
+        if (overflow) atomicOr(a.err, kErrVisit);
+        marks[0] = 0;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    for (uint32_t t = 0; t < nv; t++) {
+        const uint32_t u = __hip_atomic_load(reinterpret_cast<const uint32_t*>(vis + t), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t du = a.deg[u];
+        const uint32_t* row = a.adj + (size_t)u * a.R;
+        for (uint32_t j = lane; j < du; j += 64) {
+            const uint32_t x = row[j];
+            if (x != kEmpty) marks[x >> 5] = 0;
+        }
+    }
+    // the next search's atomics must find these words cleared
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int C>
+__global__ __launch_bounds__(64 * kSearchWaves) void buildSearchKernel(BuildArgs a) {
+    __shared__ uint64_t mkey[kSearchWaves][64 * C];
+    __shared__ unsigned char mflag[kSearchWaves][64 * C];
+    __shared__ uint32_t pend[kSearchWaves][128];
+    const int lane = laneId(), w = threadIdx.x >> 6;
+    const uint32_t wave = blockIdx.x * kSearchWaves + w;
+    uint32_t* marks = a.marks + (size_t)wave * a.markWords;
+    for (uint32_t i = wave; i < a.size; i += a.searchWaves)
+        searchOne<C>(a, i, marks, mkey[w], mflag[w], pend[w], lane);
+}
+
+// ---- robust prune (one wavefront, LDS working set) ----
+
+template <uint32_t CAP>
+struct PruneLds {
+    uint64_t keys[CAP];
+    union {
+        uint32_t list[CAP];                // reverse phase: the sources to add (read before
+        uint16_t idx[CAP];                 // the prune lists live candidates here)
+    };
+    unsigned char flags[CAP];              // 1: pruned, or the center itself
+    uint32_t out[128];
+    uint32_t rowT[128];                    // the center's current row
+};
+static_assert(sizeof(PruneLds<kSmallCap>) * kPruneWaves <= 32 * 1024, "common-tier block LDS");
+static_assert(sizeof(PruneLds<kSortCap>) <= 64 * 1024, "deferred-tier block LDS");
+
+// Distance-keyed candidates keys[0..nc) of `center` (any order, duplicates allowed,
+// nc <= CAP) -> the out-neighbours in out[0..return), as the host's robustPrune: nearest
+// first, keep a candidate unless an earlier kept s has alpha * d(s, c) <= d(center, c).
+// After each kept s the candidates still live are listed, and their rows gathered 64 at
+// a time (eight rows per 8-lane group in flight).
+template <uint32_t CAP>
+__device__ uint32_t robustPruneWave(const BuildArgs& a, uint32_t center, float alpha, PruneLds<CAP>& S,
+                                    uint32_t nc, int lane) {
+    const int sub = lane & 7, grp = lane >> 3;
+    const int normMask = (a.normsInRow && sub == 7) ? -1 : 0;
+    // bitonic sort of the keys in LDS, padded with ~0 to a power of two
+    uint32_t P = 64;
+    while (P < nc) P <<= 1;
+    for (uint32_t t = nc + lane; t < P; t += 64) S.keys[t] = ~0ull;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = lane; t < P / 2; t += 64) {
+                const uint32_t i0 = 2 * t - (t & (j - 1));
+                const uint32_t i1 = i0 + j;
+                const bool asc = (i0 & k) == 0;
+                const uint64_t x = S.keys[i0], y = S.keys[i1];
+                if ((x > y) == asc) {
+                    S.keys[i0] = y;
+                    S.keys[i1] = x;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    // drop repeats (the same id has the same distance, so repeats are adjacent and equal)
+    uint32_t nu = 0;
+    for (uint32_t b = 0; b < nc; b += 64) {
+        const uint32_t t = b + lane;
+        const uint64_t x = t < nc ? S.keys[t] : ~0ull;
+        const uint64_t prev = (t < nc && t > 0) ? S.keys[t - 1] : ~0ull;
+        const bool keep = t < nc && (t == 0 || x != prev);
+        const uint64_t m = ballot(keep);
+        __builtin_amdgcn_wave_barrier();
+        if (keep) S.keys[nu + mbcnt(m)] = x;
+        nu += (uint32_t)__popcll(m);
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (uint32_t t = lane; t < nu; t += 64) S.flags[t] = (uint32_t)S.keys[t] == center ? 1 : 0;
+    __builtin_amdgcn_wave_barrier();
+
+    uint32_t outc = 0, i = 0;
+    while (outc < a.R) {
+        int found = -1;
+        for (uint32_t b = i; b < nu && found < 0; b += 64) {
+            const uint32_t t = b + lane;
+            const uint64_t m = ballot(t < nu && S.flags[t] == 0);
+            if (m) found = (int)b + __ffsll((unsigned long long)m) - 1;
+        }
+        if (found < 0) break;
+        const uint32_t s = (uint32_t)S.keys[found];
+        if (lane == 0) S.out[outc] = s;
+        outc++;
+        if (outc >= a.R) break;
+        int4 sv = *reinterpret_cast<const int4*>(a.vec + (size_t)s * kVecBytes + sub * 16);
+        sv.w &= ~normMask;
+        const int ss = groupSum8(sliceNorm(sv));
+        uint32_t cnt = 0;
+        for (uint32_t b = (uint32_t)found + 1; b < nu; b += 64) {
+            const uint32_t t = b + lane;
+            const bool live = t < nu && S.flags[t] == 0;
+            const uint64_t m = ballot(live);
+            if (live) S.idx[cnt + mbcnt(m)] = (uint16_t)t;
+            cnt += (uint32_t)__popcll(m);
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t g0 = 0; g0 < cnt; g0 += 64) {
+            const uint32_t nb = min(64u, cnt - g0);
+            int4 v[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                v[r] = make_int4(0, 0, 0, 0);
+                const uint32_t t = (uint32_t)r * 8 + grp;
+                if (t < nb) {
+                    const uint32_t id = (uint32_t)S.keys[S.idx[g0 + t]];
+                    v[r] = *reinterpret_cast<const int4*>(a.vec + (size_t)id * kVecBytes + sub * 16);
+                }
+            }
+            int part8[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                v[r].w &= ~normMask;
+                part8[r] = slicePartial(v[r], sv);
+            }
+            const int tot = transposeReduce8(part8, lane);
+            const int mine = __shfl(tot, (lane & 7) * 8 + (lane >> 3), 64);
+            if ((uint32_t)lane < nb) {
+                const uint32_t j = S.idx[g0 + lane];
+                const int dsp = ss + mine;
+                const int dj = (int)(S.keys[j] >> 32);
+                if (alpha * (float)dsp <= (float)dj) S.flags[j] = 1;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        i = (uint32_t)found + 1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    return outc;
+}
+
+// keys[t] = (d(center, ids[t]), ids[t]) for t < count; ids in LDS.
+__device__ void keysFromIds(const BuildArgs& a, uint32_t center, const uint32_t* ids, uint32_t count,
+                            uint64_t* keys, int lane) {
+    const int sub = lane & 7, grp = lane >> 3;
+    const int normMask = (a.normsInRow && sub == 7) ? -1 : 0;
+    int4 qv = *reinterpret_cast<const int4*>(a.vec + (size_t)center * kVecBytes + sub * 16);
+    qv.w &= ~normMask;
+    const int qq = groupSum8(sliceNorm(qv));
+    for (uint32_t t0 = 0; t0 < count; t0 += 8) {
+        const uint32_t t = t0 + grp;
+        const uint32_t id = t < count ? ids[t] : 0;
+        int4 v = make_int4(0, 0, 0, 0);
+        if (t < count) v = *reinterpret_cast<const int4*>(a.vec + (size_t)id * kVecBytes + sub * 16);
+        v.w &= ~normMask;
+        const int d = qq + groupSum8(slicePartial(v, qv));
+        if (t < count && sub == 0) keys[t] = keyOf(d, id);
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Each batch point: its expanded nodes plus its current row, pruned to its new row.
+// items == nullptr: every batch index, wider prunes deferred; else the deferred indices.
+template <uint32_t CAP, int WPB>
+__global__ __launch_bounds__(64 * WPB) void buildPruneKernel(BuildArgs a, float alpha, const uint32_t* items) {
+    __shared__ PruneLds<CAP> lds[WPB];
+    const int lane = laneId(), w = threadIdx.x >> 6;
+    PruneLds<CAP>& S = lds[w];
+    const uint32_t count = items ? *a.ndeferred : a.size;
+    for (uint32_t it = blockIdx.x * WPB + w; it < count; it += gridDim.x * WPB) {
+        const uint32_t i = items ? items[it] : it;
+        const uint32_t p = a.batch[i];
+        const uint32_t nv = a.nvisited[i];
+        uint64_t* pr = a.pairs + (size_t)i * a.R;
+        if (nv > kVisitCap) {
+            for (uint32_t t = lane; t < a.R; t += 64) pr[t] = ~0ull;
+            continue;
+        }
+        const uint32_t dp = a.deg[p];
+        const uint32_t nc = nv + dp;
+        if (nc > CAP) {
+            if (items == nullptr) {
+                if (lane == 0) a.deferred[atomicAdd(a.ndeferred, 1u)] = i;
+            } else {
+                if (lane == 0) atomicOr(a.err, kErrPrune);
+                for (uint32_t t = lane; t < a.R; t += 64) pr[t] = ~0ull;
+            }
+            continue;
+        }
+        for (uint32_t t = lane; t < nv; t += 64) S.keys[t] = a.visited[(size_t)i * kVisitCap + t];
+        for (uint32_t t = lane; t < dp; t += 64) S.rowT[t] = a.adj[(size_t)p * a.R + t];
+        __builtin_amdgcn_wave_barrier();
+        keysFromIds(a, p, S.rowT, dp, S.keys + nv, lane);
+        const uint32_t outc = robustPruneWave<CAP>(a, p, alpha, S, nc, lane);
+        for (uint32_t t = lane; t < a.R; t += 64) {
+            const uint32_t x = t < outc ? S.out[t] : kEmpty;
+            a.adj[(size_t)p * a.R + t] = x;
+            pr[t] = t < outc ? (((uint64_t)x << 32) | p) : ~0ull;
+        }
+        if (lane == 0) a.deg[p] = outc;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// First index of every target's run in the sorted edge list.
+__global__ void buildHeadsKernel(const uint64_t* sorted, uint32_t npairs, uint32_t* heads, uint32_t* nheads) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npairs) return;
+    const uint64_t x = sorted[i];
+    if (x != ~0ull && (i == 0 || (sorted[i - 1] >> 32) != (x >> 32))) heads[atomicAdd(nheads, 1u)] = i;
+}
+
+// Each target with new in-edges: absent sources appended in source order, or, past R,
+// the row and those sources robust-pruned together.  items == nullptr: every run, runs
+// whose prune is wider than CAP deferred; else the deferred head indices.
+template <uint32_t CAP, int WPB>
+__global__ __launch_bounds__(64 * WPB) void buildReverseKernel(BuildArgs a, float alpha, const uint32_t* items) {
+    __shared__ PruneLds<CAP> lds[WPB];
+    const int lane = laneId(), w = threadIdx.x >> 6;
+    PruneLds<CAP>& S = lds[w];
+    const uint32_t count = items ? *a.ndeferred : *a.nheads;
+    for (uint32_t it = blockIdx.x * WPB + w; it < count; it += gridDim.x * WPB) {
+        const uint32_t h = items ? items[it] : it;
+        const uint32_t start = a.heads[h];
+        const uint32_t t = (uint32_t)(a.sorted[start] >> 32);
+        const uint32_t dt = a.deg[t];
+        for (uint32_t r = lane; r < dt; r += 64) S.rowT[r] = a.adj[(size_t)t * a.R + r];
+        __builtin_amdgcn_wave_barrier();
+        uint32_t nadd = 0;
+        for (uint32_t b = start;; b += 64) {
+            const uint32_t idx = b + lane;
+            const uint64_t x = idx < a.npairs ? a.sorted[idx] : ~0ull;
+            const bool inRun = x != ~0ull && (uint32_t)(x >> 32) == t;
+            const uint32_t s = (uint32_t)x;
+            bool present = false;
+            if (inRun)
+                for (uint32_t r = 0; r < dt; r++) present |= S.rowT[r] == s;
+            const bool keep = inRun && !present && s != t;
+            const uint64_t km = ballot(keep);
+            const uint32_t pos = nadd + mbcnt(km);
+            if (keep && pos < CAP) S.list[pos] = s;
+            nadd += (uint32_t)__popcll(km);
+            if (ballot(inRun) != ~0ull) break;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (nadd == 0) continue;
+        if (dt + nadd <= a.R) {
+            for (uint32_t r = lane; r < nadd; r += 64) a.adj[(size_t)t * a.R + dt + r] = S.list[r];
+            if (lane == 0) a.deg[t] = dt + nadd;
+            continue;
+        }
+        if (nadd + dt > CAP) {
+            if (lane == 0) {
+                if (items == nullptr) a.deferred[atomicAdd(a.ndeferred, 1u)] = h;
+                else atomicOr(a.err, kErrPrune);
+            }
+            continue;
+        }
+        keysFromIds(a, t, S.list, nadd, S.keys, lane);
+        keysFromIds(a, t, S.rowT, dt, S.keys + nadd, lane);
+        const uint32_t outc = robustPruneWave<CAP>(a, t, alpha, S, nadd + dt, lane);
+        for (uint32_t r = lane; r < a.R; r += 64) a.adj[(size_t)t * a.R + r] = r < outc ? S.out[r] : kEmpty;
+        if (lane == 0) a.deg[t] = outc;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+struct DevMem {
+    std::vector<void*> ptrs;
+    ~DevMem() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    hipError_t alloc(T** p, size_t count) {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T));
+        if (e == hipSuccess) ptrs.push_back(q);
+        *p = static_cast<T*>(q);
+        return e;
+    }
+};
+
+#define BUILD_TRY(x)                        \
+    do {                                    \
+        const hipError_t e_ = (x);          \
+        if (e_ != hipSuccess) return e_;    \
+    } while (0)
+
+hipError_t launchSearch(const BuildArgs& a, int C, uint32_t blocks, hipStream_t s) {
+    switch (C) {
+        case 1: hipLaunchKernelGGL((buildSearchKernel<1>), dim3(blocks), dim3(64 * kSearchWaves), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((buildSearchKernel<2>), dim3(blocks), dim3(64 * kSearchWaves), 0, s, a); break;
+        default: hipLaunchKernelGGL((buildSearchKernel<4>), dim3(blocks), dim3(64 * kSearchWaves), 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// Builds the graph over the n device rows of `vec` into host arrays outDeg (n) and
+// outAdj (n x R, kEmpty padded).  *status: 0, or kErrVisit / kErrPrune when a search
+// expanded more than kVisitCap nodes or a prune met more than kSortCap candidates.
+hipError_t launchBuildVamana(const int8_t* vec, bool normsInRow, uint32_t n, uint32_t R, uint32_t L,
+                             float alpha, bool twoPass, uint64_t seed, int cus, uint32_t* outDeg,
+                             uint32_t* outAdj, uint32_t* status, hipStream_t s) {
+    *status = 0;
+    const std::vector<uint32_t> order = tools::permutation(n, tools::mixSeed(seed, 1));
+    const uint32_t maxBatch = std::max<uint32_t>(1, (uint32_t)(0.02 * n));
+    const uint32_t markWords = (n + 31) / 32;
+    const int C = L <= 64 ? 1 : L <= 128 ? 2 : 4;
+    // up to six search wavefronts per SIMD, within the bitmap budget
+    uint32_t searchBlocksMax = (uint32_t)std::max(1, cus) * 6;
+    while (searchBlocksMax > 1 && (size_t)searchBlocksMax * kSearchWaves * markWords * 4 > kMarkBudget)
+        searchBlocksMax /= 2;
+    const uint32_t searchWavesMax = searchBlocksMax * kSearchWaves;
+    const uint32_t pruneBlocks = (uint32_t)std::max(1, cus) * 5;
+    const uint32_t wideBlocks = (uint32_t)std::max(1, cus);
+
+    DevMem m;
+    uint32_t *adj, *deg, *dOrder, *nvisited, *marks, *heads, *nheads, *deferred, *ndeferred, *err;
+    uint64_t *visited, *pairs, *sorted;
+    BUILD_TRY(m.alloc(&adj, (size_t)n * R));
+    BUILD_TRY(m.alloc(&deg, n));
+    BUILD_TRY(m.alloc(&dOrder, n));
+    BUILD_TRY(m.alloc(&nvisited, maxBatch));
+    BUILD_TRY(m.alloc(&visited, (size_t)maxBatch * kVisitCap));
+    BUILD_TRY(m.alloc(&marks, (size_t)searchWavesMax * markWords));
+    BUILD_TRY(m.alloc(&pairs, (size_t)maxBatch * R));
+    BUILD_TRY(m.alloc(&sorted, (size_t)maxBatch * R));
+    BUILD_TRY(m.alloc(&heads, (size_t)maxBatch * R));
+    BUILD_TRY(m.alloc(&deferred, (size_t)maxBatch * R));
+    BUILD_TRY(m.alloc(&nheads, 1));
+    BUILD_TRY(m.alloc(&ndeferred, 1));
+    BUILD_TRY(m.alloc(&err, 1));
+    size_t tempBytes = 0;
+    BUILD_TRY(rocprim::radix_sort_keys(nullptr, tempBytes, pairs, sorted, (size_t)maxBatch * R, 0, 64, s));
+    unsigned char* temp;
+    BUILD_TRY(m.alloc(&temp, tempBytes));
+    BUILD_TRY(hipMemsetAsync(adj, 0xFF, (size_t)n * R * 4, s));
+    BUILD_TRY(hipMemsetAsync(deg, 0, (size_t)n * 4, s));
+    BUILD_TRY(hipMemsetAsync(marks, 0, (size_t)searchWavesMax * markWords * 4, s));
+    BUILD_TRY(hipMemsetAsync(err, 0, 4, s));
+    BUILD_TRY(hipMemcpyAsync(dOrder, order.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+
+    BuildArgs a = {};
+    a.vec = vec;
+    a.normsInRow = normsInRow ? 1 : 0;
+    a.n = n;
+    a.R = R;
+    a.L = L;
+    a.adj = adj;
+    a.deg = deg;
+    a.visited = visited;
+    a.nvisited = nvisited;
+    a.marks = marks;
+    a.markWords = markWords;
+    a.pairs = pairs;
+    a.sorted = sorted;
+    a.heads = heads;
+    a.nheads = nheads;
+    a.deferred = deferred;
+    a.ndeferred = ndeferred;
+    a.err = err;
+
+    for (int pass = twoPass ? 0 : 1; pass < 2; pass++) {
+        const float al = pass == 0 ? 1.0f : alpha;
+        uint32_t count = 0, bsize = 1;
+        while (count < n) {
+            const uint32_t size = std::min(bsize, n - count);
+            bsize = std::min(maxBatch, bsize * 2);
+            a.batch = dOrder + count;
+            a.size = size;
+            count += size;
+            const uint32_t blocks = std::min(searchBlocksMax, (size + kSearchWaves - 1) / kSearchWaves);
+            a.searchWaves = blocks * kSearchWaves;
+            BUILD_TRY(launchSearch(a, C, blocks, s));
+            // prunes of the batch points, the wide ones after the others
+            BUILD_TRY(hipMemsetAsync(ndeferred, 0, 4, s));
+            const uint32_t pb = std::min(pruneBlocks, (size + kPruneWaves - 1) / kPruneWaves);
+            hipLaunchKernelGGL((buildPruneKernel<kSmallCap, kPruneWaves>), dim3(pb), dim3(64 * kPruneWaves), 0,
+                               s, a, al, nullptr);
+            BUILD_TRY(hipGetLastError());
+            hipLaunchKernelGGL((buildPruneKernel<kSortCap, 1>), dim3(wideBlocks), dim3(64), 0, s, a, al,
+                               static_cast<const uint32_t*>(deferred));
+            BUILD_TRY(hipGetLastError());
+            // new edges grouped by target, then the reverse phase
+            a.npairs = size * R;
+            size_t tb = tempBytes;
+            BUILD_TRY(rocprim::radix_sort_keys(temp, tb, pairs, sorted, (size_t)a.npairs, 0, 64, s));
+            BUILD_TRY(hipMemsetAsync(nheads, 0, 4, s));
+            BUILD_TRY(hipMemsetAsync(ndeferred, 0, 4, s));
+            hipLaunchKernelGGL(buildHeadsKernel, dim3((a.npairs + 255) / 256), dim3(256), 0, s, sorted,
+                               a.npairs, heads, nheads);
+            BUILD_TRY(hipGetLastError());
+            hipLaunchKernelGGL((buildReverseKernel<kSmallCap, kPruneWaves>), dim3(pruneBlocks),
+                               dim3(64 * kPruneWaves), 0, s, a, al, nullptr);
+            BUILD_TRY(hipGetLastError());
+            hipLaunchKernelGGL((buildReverseKernel<kSortCap, 1>), dim3(wideBlocks), dim3(64), 0, s, a, al,
+                               static_cast<const uint32_t*>(deferred));
+            BUILD_TRY(hipGetLastError());
+        }
+    }
+    BUILD_TRY(hipMemcpyAsync(outDeg, deg, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    BUILD_TRY(hipMemcpyAsync(outAdj, adj, (size_t)n * R * 4, hipMemcpyDeviceToHost, s));
+    BUILD_TRY(hipMemcpyAsync(status, err, 4, hipMemcpyDeviceToHost, s));
+    return hipStreamSynchronize(s);
+}
+
+}  // namespace pann

@@ -787,6 +787,27 @@ int pann_bruteforce_knn(const pann_vectors* base, const pann_vectors* queries, u
     return PANN_OK;
 }
 
+int pann_build_vamana_gpu(const pann_vectors* base, uint32_t R, uint32_t L, float alpha,
+                          int two_pass, uint64_t seed, uint32_t* out_degrees,
+                          uint32_t* out_edges, void* stream) {
+    if (!base || !out_degrees || !out_edges) return setError(PANN_ERR_ARG, "NULL argument");
+    if (base->n == 0) return setError(PANN_ERR_ARG, "no points");
+    if (R == 0 || R > 128) return setError(PANN_ERR_CAPACITY, "R=%u outside 1..128", R);
+    if (L == 0 || L > 256) return setError(PANN_ERR_CAPACITY, "L=%u outside 1..256", L);
+    if (alpha != alpha) return setError(PANN_ERR_ARG, "alpha is NaN");
+    int cus = 0;
+    int r = deviceCUs(&cus);
+    if (r != PANN_OK) return r;
+    uint32_t status = 0;
+    const hipError_t e = launchBuildVamana(base->dev, base->dim <= (uint32_t)kNormInRowMaxDim, base->n,
+                                           R, L, alpha, two_pass != 0, seed, cus, out_degrees,
+                                           out_edges, &status, (hipStream_t)stream);
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "Vamana build: %s", hipGetErrorString(e));
+    if (status & 1u) return setError(PANN_ERR_CAPACITY, "a build search expanded more than 1024 nodes");
+    if (status & 2u) return setError(PANN_ERR_CAPACITY, "a robust prune met more than 2048 candidates");
+    return PANN_OK;
+}
+
 void pann_searcher_free(pann_searcher* s) {
     if (!s) return;
     freeOutputs(s->out);

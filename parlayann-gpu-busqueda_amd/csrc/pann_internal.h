@@ -270,6 +270,11 @@ hipError_t launchGatherBench(const int8_t* rows, uint32_t nrows, const int8_t* q
 // dst row i = src row idx[i] (kVecBytes rows); indices >= nsrc leave the row untouched.
 hipError_t launchGatherRows(int8_t* dst, const int8_t* src, const uint32_t* idx, uint32_t n,
                             uint32_t nsrc, hipStream_t s);
+// Vamana graph construction (build_gpu.hip) into host arrays; *status 1: a search
+// expanded too many nodes, 2: a prune met too many candidates.
+hipError_t launchBuildVamana(const int8_t* vec, bool normsInRow, uint32_t n, uint32_t R, uint32_t L,
+                             float alpha, bool twoPass, uint64_t seed, int cus, uint32_t* outDeg,
+                             uint32_t* outAdj, uint32_t* status, hipStream_t s);
 hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_t n,
                                const int8_t* queries, const int32_t* qnorm, uint32_t nq,
                                uint32_t k, bool normsInRow, uint32_t* outIds, int32_t* outDists,

@@ -29,6 +29,18 @@ def build_vamana(base, R=32, L=64, alpha=1.2, two_pass=True, seed=7, nthreads=No
     return formats.Graph.from_dense(deg, dense, R)
 
 
+def build_vamana_gpu(vectors, R=32, L=64, alpha=1.2, two_pass=True, seed=7):
+    """build_vamana's graph, built on the device from the rows of a search.Vectors
+    (pann_build_vamana_gpu); equal to build_vamana's for the same arguments."""
+    from ._lib import check, lib
+    n = vectors.n
+    deg = np.zeros(n, dtype=np.uint32)
+    dense = np.zeros((n, R), dtype=np.uint32)
+    check(lib().pann_build_vamana_gpu(vectors.handle, R, L, float(alpha), int(two_pass), seed,
+                                      _ptr(deg), _ptr(dense), None))
+    return formats.Graph.from_dense(deg, dense, R)
+
+
 def _via_file(fn, suffix):
     fd, path = tempfile.mkstemp(suffix=suffix)
     os.close(fd)

@@ -1,0 +1,55 @@
+"""GPU Vamana construction (pann_build_vamana_gpu) against the host builder
+(pann_build_vamana, the batch insertion of ParlayANN's Vamana): the same degrees and the
+same rows, entry for entry, for the same arguments.  Cases cover one point, frontier
+widths of one, two and four registers (L 16 / 100 / 200), rows wider than 64 (R = 100),
+rows with and without their norm embedded (dim 100 vs 128), single and two-pass builds,
+and duplicated points (distance-0 ties broken by id)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # n, dim, R, L, alpha, two_pass, seed
+    (1, 16, 8, 8, 1.2, True, 1),
+    (300, 16, 8, 16, 1.2, True, 3),
+    (3000, 100, 32, 64, 1.2, True, 7),
+    (3000, 128, 24, 100, 1.3, False, 5),
+    (6000, 37, 64, 200, 1.2, True, 11),
+    (2500, 32, 100, 128, 1.1, True, 2),
+]
+
+
+def _data(n, dim, seed):
+    from pann import datagen
+    base, _ = datagen.gmm_dataset(n, 1, dim=dim, ncomp=max(1, n // 200), seeds=(seed, seed + 1, seed + 2))
+    if n >= 100:
+        base[n // 2: n // 2 + 10] = base[:10]   # duplicated points
+    return base
+
+
+@pytest.mark.parametrize("n,dim,R,L,alpha,two_pass,seed", CASES)
+def test_gpu_build_equals_host(gpu, n, dim, R, L, alpha, two_pass, seed):
+    from pann import builders, search
+    base = _data(n, dim, seed)
+    print(f"[build] n={n} dim={dim} R={R} L={L} alpha={alpha} two_pass={two_pass}", flush=True)
+    host = builders.build_vamana(base, R=R, L=L, alpha=alpha, two_pass=two_pass, seed=seed, nthreads=8)
+    b = search.Vectors(base)
+    try:
+        g = builders.build_vamana_gpu(b, R=R, L=L, alpha=alpha, two_pass=two_pass, seed=seed)
+    finally:
+        b.free()
+    assert np.array_equal(g.degrees, host.degrees)
+    assert np.array_equal(g.edges, host.edges)
+
+
+def test_gpu_build_rejects_bad_arguments(gpu):
+    from pann import PannError, builders, search
+    b = search.Vectors(np.zeros((4, 8), np.int8))
+    try:
+        with pytest.raises(PannError):
+            builders.build_vamana_gpu(b, R=129, L=16)
+        with pytest.raises(PannError):
+            builders.build_vamana_gpu(b, R=8, L=257)
+    finally:
+        b.free()
