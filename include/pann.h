@@ -325,6 +325,16 @@ int pann_bruteforce_knn(const pann_vectors* base, const pann_vectors* queries, u
 int pann_build_vamana_gpu(const pann_vectors* base, uint32_t R, uint32_t L, float alpha,
                           int two_pass, uint64_t seed, uint32_t* out_degrees,
                           uint32_t* out_edges, void* stream);
+/* HNSW graph over the rows of `base`, built on the device and written as an HNSW model v3
+ * file (the format HnswGPU loads, cuda/hnsw_lector.cpp:28-111): node levels
+ * floor(-ln U * ml), M neighbours per upper layer and 2M on layer 0, prefix-doubling batch
+ * insertion (greedy descent to the node's level, efc-wide search per layer below,
+ * neighbour heuristic with alpha, reverse edges grouped by layer and target), the
+ * reference's next step for HNSW (README.md:350).  The file equals the one pann_build_hnsw
+ * (include/pann_tools.h) writes for the same arguments, byte for byte.  M <= 64,
+ * efc <= 256; PANN_ERR_CAPACITY if a search expands more than 1024 nodes.  Blocking. */
+int pann_build_hnsw_gpu(const pann_vectors* base, uint32_t M, uint32_t efc, float ml, float alpha,
+                        uint64_t seed, const char* out_path, void* stream);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,7 @@
 // each, and the rare wider prunes (up to 2048 candidates) deferred to a second launch
 // with one wavefront per block.
 #include "device_common.h"
+#include "hnsw_plan.h"
 #include "tools_common.h"
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -148,30 +149,77 @@ __device__ void mergeFrontier(uint64_t (&fk)[C], uint64_t (&ex)[C], uint32_t& F,
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int C>
-__device__ void searchOne(const BuildArgs& a, uint32_t i, uint32_t* marks, uint64_t* mkey,
-                          unsigned char* mflag, uint32_t* pend, int lane) {
-    uint64_t* vis = a.visited + (size_t)i * kVisitCap;
-    const uint32_t p = a.batch[i];
-    const int sub = lane & 7, grp = lane >> 3;
-    const int normMask = (a.normsInRow && sub == 7) ? -1 : 0;
-    int4 qv = *reinterpret_cast<const int4*>(a.vec + (size_t)p * kVecBytes + sub * 16);
-    qv.w &= ~normMask;
-    const int qq = groupSum8(sliceNorm(qv));
+// Rows of one graph layer: node u's row is adj + index(u) * width, its length deg[index(u)];
+// index(u) = u, or, for an HNSW upper layer, upStart[u] + layer - 1.
+struct LayerRows {
+    const uint32_t* adj;
+    const uint32_t* deg;
+    uint32_t width;
+    const uint32_t* upStart;  // nullptr: index(u) = u
+    uint32_t layer;
+    __device__ uint32_t index(uint32_t u) const { return upStart ? upStart[u] + layer - 1 : u; }
+};
 
-    uint64_t fk[C], ex[C];
+// Marks the ids pend[0..np) already known fresh, scores them against the query slice and
+// merges them into the frontier.
+template <int C>
+__device__ void scoreAndMerge(const int8_t* vec, const uint32_t* pend, uint32_t np, const int4& qv, int qq,
+                              int normMask, uint64_t (&fk)[C], uint64_t (&ex)[C], uint32_t& F, uint32_t L,
+                              uint64_t* mkey, unsigned char* mflag, int lane) {
+    const int sub = lane & 7, grp = lane >> 3;
+    for (uint32_t b0 = 0; b0 < np; b0 += 64) {
+        const uint32_t nb = min(64u, np - b0);
+        int4 v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            v[r] = make_int4(0, 0, 0, 0);
+            const uint32_t t = (uint32_t)r * 8 + grp;
+            if (t < nb) v[r] = *reinterpret_cast<const int4*>(vec + (size_t)pend[b0 + t] * kVecBytes + sub * 16);
+        }
+        int part8[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            v[r].w &= ~normMask;
+            part8[r] = slicePartial(v[r], qv);
+        }
+        const int tot = transposeReduce8(part8, lane);
+        const int mine = __shfl(tot, (lane & 7) * 8 + (lane >> 3), 64);
+        const uint64_t ck = (uint32_t)lane < nb ? keyOf(qq + mine, pend[b0 + lane]) : ~0ull;
+        mergeFrontier<C>(fk, ex, F, ck, L, mkey, mflag, lane);
+    }
+}
+
+// The host builders' search (BuildSearcher::search, tools_common.h) on one layer: starts
+// (LDS, nstarts ids) marked and kept as the best L, then hops until every frontier entry
+// is expanded.  The expanded keys go to vis[0..*nv) (HBM); *nv = visCap + 1 on overflow.
+// On return the frontier is fk (F entries) and every bitmap word the search set is clear.
+template <int C>
+__device__ uint32_t exactSearch(const int8_t* vec, const LayerRows& g, const uint32_t* starts,
+                                uint32_t nstarts, uint32_t L, const int4& qv, int qq, int normMask,
+                                uint32_t* marks, uint64_t* mkey, unsigned char* mflag, uint32_t* pend,
+                                uint64_t* vis, uint32_t visCap, uint64_t (&fk)[C], uint32_t* nvOut, int lane) {
+    uint64_t ex[C];
 #pragma unroll
     for (int c = 0; c < C; c++) {
         fk[c] = ~0ull;
         ex[c] = 0;
     }
-    // the start node 0
-    if (lane == 0) atomicOr(&marks[0], 1u);
-    {
-        const int d0 = readlane(groupDistance(a.vec, 0, sub, qv, qq, normMask), 0);
-        if (lane == 0) fk[0] = keyOf(d0, 0);
+    uint32_t F = 0;
+    for (uint32_t b0 = 0; b0 < nstarts; b0 += 64) {
+        const uint32_t t = b0 + lane;
+        const uint32_t x = t < nstarts ? starts[t] : kEmpty;
+        bool fresh = false;
+        if (x != kEmpty) {
+            const uint32_t bit = 1u << (x & 31);
+            fresh = (atomicOr(&marks[x >> 5], bit) & bit) == 0;
+        }
+        const uint64_t fm = ballot(fresh);
+        if (fresh) pend[mbcnt(fm)] = x;
+        __builtin_amdgcn_wave_barrier();
+        scoreAndMerge<C>(vec, pend, (uint32_t)__popcll(fm), qv, qq, normMask, fk, ex, F, L, mkey, mflag, lane);
+        __builtin_amdgcn_wave_barrier();
     }
-    uint32_t F = 1, nv = 0;
+    uint32_t nv = 0;
     bool overflow = false;
     while (true) {
         int e = -1;
@@ -181,7 +229,7 @@ __device__ void searchOne(const BuildArgs& a, uint32_t i, uint32_t* marks, uint6
             if (e < 0 && m) e = c * 64 + __ffsll((unsigned long long)m) - 1;
         }
         if (e < 0) break;
-        if (nv >= kVisitCap) {
+        if (nv >= visCap) {
             overflow = true;
             break;
         }
@@ -195,8 +243,9 @@ __device__ void searchOne(const BuildArgs& a, uint32_t i, uint32_t* marks, uint6
         if (lane == 0) vis[nv] = cur;
         nv++;
         const uint32_t u = (uint32_t)cur;
-        const uint32_t du = a.deg[u];
-        const uint32_t* row = a.adj + (size_t)u * a.R;
+        const uint32_t ri = g.index(u);
+        const uint32_t du = g.deg[ri];
+        const uint32_t* row = g.adj + (size_t)ri * g.width;
         uint32_t np = 0;
         for (uint32_t ch = 0; ch * 64 < du; ch++) {
             const uint32_t j = ch * 64 + lane;
@@ -211,41 +260,19 @@ __device__ void searchOne(const BuildArgs& a, uint32_t i, uint32_t* marks, uint6
             np += (uint32_t)__popcll(fm);
         }
         __builtin_amdgcn_wave_barrier();
-        for (uint32_t b0 = 0; b0 < np; b0 += 64) {
-            const uint32_t nb = min(64u, np - b0);
-            int4 v[8];
-#pragma unroll
-            for (int r = 0; r < 8; r++) {
-                v[r] = make_int4(0, 0, 0, 0);
-                const uint32_t t = (uint32_t)r * 8 + grp;
-                if (t < nb)
-                    v[r] = *reinterpret_cast<const int4*>(a.vec + (size_t)pend[b0 + t] * kVecBytes + sub * 16);
-            }
-            int part8[8];
-#pragma unroll
-            for (int r = 0; r < 8; r++) {
-                v[r].w &= ~normMask;
-                part8[r] = slicePartial(v[r], qv);
-            }
-            const int tot = transposeReduce8(part8, lane);
-            const int mine = __shfl(tot, (lane & 7) * 8 + (lane >> 3), 64);
-            const uint64_t ck = (uint32_t)lane < nb ? keyOf(qq + mine, pend[b0 + lane]) : ~0ull;
-            mergeFrontier<C>(fk, ex, F, ck, a.L, mkey, mflag, lane);
-        }
+        scoreAndMerge<C>(vec, pend, np, qv, qq, normMask, fk, ex, F, L, mkey, mflag, lane);
     }
-    // clear the bitmap words this search set: node 0's and those of the expanded rows,
-    // whose ids are read back past the vector L1 (it does not see this wave's stores)
-    if (lane == 0) {
-        a.nvisited[i] = overflow ? kVisitCap + 1 : nv;
-        if (overflow) atomicOr(a.err, kErrVisit);
-        marks[0] = 0;
-    }
+    // clear the bitmap words this search set: those of the starts and of the expanded
+    // rows, whose ids are read back past the vector L1 (it does not see this wave's stores)
+    *nvOut = overflow ? visCap + 1 : nv;
+    for (uint32_t t = lane; t < nstarts; t += 64) marks[starts[t] >> 5] = 0;
     __builtin_amdgcn_s_waitcnt(0);
     for (uint32_t t = 0; t < nv; t++) {
         const uint32_t u = __hip_atomic_load(reinterpret_cast<const uint32_t*>(vis + t), __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t du = a.deg[u];
-        const uint32_t* row = a.adj + (size_t)u * a.R;
+        const uint32_t ri = g.index(u);
+        const uint32_t du = g.deg[ri];
+        const uint32_t* row = g.adj + (size_t)ri * g.width;
         for (uint32_t j = lane; j < du; j += 64) {
             const uint32_t x = row[j];
             if (x != kEmpty) marks[x >> 5] = 0;
@@ -254,18 +281,38 @@ __device__ void searchOne(const BuildArgs& a, uint32_t i, uint32_t* marks, uint6
     // the next search's atomics must find these words cleared
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
+    return F;
 }
 
+// Vamana: each batch point's search from node 0; its expanded nodes are the prune's input.
 template <int C>
 __global__ __launch_bounds__(64 * kSearchWaves) void buildSearchKernel(BuildArgs a) {
     __shared__ uint64_t mkey[kSearchWaves][64 * C];
     __shared__ unsigned char mflag[kSearchWaves][64 * C];
     __shared__ uint32_t pend[kSearchWaves][128];
+    __shared__ uint32_t start[kSearchWaves][1];
     const int lane = laneId(), w = threadIdx.x >> 6;
     const uint32_t wave = blockIdx.x * kSearchWaves + w;
     uint32_t* marks = a.marks + (size_t)wave * a.markWords;
-    for (uint32_t i = wave; i < a.size; i += a.searchWaves)
-        searchOne<C>(a, i, marks, mkey[w], mflag[w], pend[w], lane);
+    const LayerRows g{a.adj, a.deg, a.R, nullptr, 0};
+    const int sub = lane & 7;
+    const int normMask = (a.normsInRow && sub == 7) ? -1 : 0;
+    if (lane == 0) start[w][0] = 0;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = wave; i < a.size; i += a.searchWaves) {
+        const uint32_t p = a.batch[i];
+        int4 qv = *reinterpret_cast<const int4*>(a.vec + (size_t)p * kVecBytes + sub * 16);
+        qv.w &= ~normMask;
+        const int qq = groupSum8(sliceNorm(qv));
+        uint64_t fk[C];
+        uint32_t nv = 0;
+        exactSearch<C>(a.vec, g, start[w], 1, a.L, qv, qq, normMask, marks, mkey[w], mflag[w], pend[w],
+                       a.visited + (size_t)i * kVisitCap, kVisitCap, fk, &nv, lane);
+        if (lane == 0) {
+            a.nvisited[i] = nv;
+            if (nv > kVisitCap) atomicOr(a.err, kErrVisit);
+        }
+    }
 }
 
 // ---- robust prune (one wavefront, LDS working set) ----
@@ -285,15 +332,16 @@ static_assert(sizeof(PruneLds<kSmallCap>) * kPruneWaves <= 32 * 1024, "common-ti
 static_assert(sizeof(PruneLds<kSortCap>) <= 64 * 1024, "deferred-tier block LDS");
 
 // Distance-keyed candidates keys[0..nc) of `center` (any order, duplicates allowed,
-// nc <= CAP) -> the out-neighbours in out[0..return), as the host's robustPrune: nearest
-// first, keep a candidate unless an earlier kept s has alpha * d(s, c) <= d(center, c).
-// After each kept s the candidates still live are listed, and their rows gathered 64 at
-// a time (eight rows per 8-lane group in flight).
-template <uint32_t CAP>
-__device__ uint32_t robustPruneWave(const BuildArgs& a, uint32_t center, float alpha, PruneLds<CAP>& S,
-                                    uint32_t nc, int lane) {
+// nc <= CAP) -> at most m out-neighbours in out[0..return), nearest first: a candidate is
+// kept unless an earlier kept s has alpha * d(s, c) <= d(center, c) (Vamana's robustPrune)
+// or, STRICT, alpha * d(s, c) < d(center, c) (HNSW's neighbour heuristic).  After each kept
+// s the candidates still live are listed, and their rows gathered 64 at a time (eight
+// rows per 8-lane group in flight).
+template <uint32_t CAP, bool STRICT>
+__device__ uint32_t robustPruneWave(const int8_t* vec, int normsInRow, uint32_t m, uint32_t center,
+                                    float alpha, PruneLds<CAP>& S, uint32_t nc, int lane) {
     const int sub = lane & 7, grp = lane >> 3;
-    const int normMask = (a.normsInRow && sub == 7) ? -1 : 0;
+    const int normMask = (normsInRow && sub == 7) ? -1 : 0;
     // bitonic sort of the keys in LDS, padded with ~0 to a power of two
     uint32_t P = 64;
     while (P < nc) P <<= 1;
@@ -331,7 +379,7 @@ __device__ uint32_t robustPruneWave(const BuildArgs& a, uint32_t center, float a
     __builtin_amdgcn_wave_barrier();
 
     uint32_t outc = 0, i = 0;
-    while (outc < a.R) {
+    while (outc < m) {
         int found = -1;
         for (uint32_t b = i; b < nu && found < 0; b += 64) {
             const uint32_t t = b + lane;
@@ -342,8 +390,8 @@ __device__ uint32_t robustPruneWave(const BuildArgs& a, uint32_t center, float a
         const uint32_t s = (uint32_t)S.keys[found];
         if (lane == 0) S.out[outc] = s;
         outc++;
-        if (outc >= a.R) break;
-        int4 sv = *reinterpret_cast<const int4*>(a.vec + (size_t)s * kVecBytes + sub * 16);
+        if (outc >= m) break;
+        int4 sv = *reinterpret_cast<const int4*>(vec + (size_t)s * kVecBytes + sub * 16);
         sv.w &= ~normMask;
         const int ss = groupSum8(sliceNorm(sv));
         uint32_t cnt = 0;
@@ -364,7 +412,7 @@ __device__ uint32_t robustPruneWave(const BuildArgs& a, uint32_t center, float a
                 const uint32_t t = (uint32_t)r * 8 + grp;
                 if (t < nb) {
                     const uint32_t id = (uint32_t)S.keys[S.idx[g0 + t]];
-                    v[r] = *reinterpret_cast<const int4*>(a.vec + (size_t)id * kVecBytes + sub * 16);
+                    v[r] = *reinterpret_cast<const int4*>(vec + (size_t)id * kVecBytes + sub * 16);
                 }
             }
             int part8[8];
@@ -379,7 +427,8 @@ __device__ uint32_t robustPruneWave(const BuildArgs& a, uint32_t center, float a
                 const uint32_t j = S.idx[g0 + lane];
                 const int dsp = ss + mine;
                 const int dj = (int)(S.keys[j] >> 32);
-                if (alpha * (float)dsp <= (float)dj) S.flags[j] = 1;
+                const float lhs = alpha * (float)dsp;
+                if (STRICT ? lhs < (float)dj : lhs <= (float)dj) S.flags[j] = 1;
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -390,18 +439,18 @@ __device__ uint32_t robustPruneWave(const BuildArgs& a, uint32_t center, float a
 }
 
 // keys[t] = (d(center, ids[t]), ids[t]) for t < count; ids in LDS.
-__device__ void keysFromIds(const BuildArgs& a, uint32_t center, const uint32_t* ids, uint32_t count,
-                            uint64_t* keys, int lane) {
+__device__ void keysFromIds(const int8_t* vec, int normsInRow, uint32_t center, const uint32_t* ids,
+                            uint32_t count, uint64_t* keys, int lane) {
     const int sub = lane & 7, grp = lane >> 3;
-    const int normMask = (a.normsInRow && sub == 7) ? -1 : 0;
-    int4 qv = *reinterpret_cast<const int4*>(a.vec + (size_t)center * kVecBytes + sub * 16);
+    const int normMask = (normsInRow && sub == 7) ? -1 : 0;
+    int4 qv = *reinterpret_cast<const int4*>(vec + (size_t)center * kVecBytes + sub * 16);
     qv.w &= ~normMask;
     const int qq = groupSum8(sliceNorm(qv));
     for (uint32_t t0 = 0; t0 < count; t0 += 8) {
         const uint32_t t = t0 + grp;
         const uint32_t id = t < count ? ids[t] : 0;
         int4 v = make_int4(0, 0, 0, 0);
-        if (t < count) v = *reinterpret_cast<const int4*>(a.vec + (size_t)id * kVecBytes + sub * 16);
+        if (t < count) v = *reinterpret_cast<const int4*>(vec + (size_t)id * kVecBytes + sub * 16);
         v.w &= ~normMask;
         const int d = qq + groupSum8(slicePartial(v, qv));
         if (t < count && sub == 0) keys[t] = keyOf(d, id);
@@ -440,8 +489,8 @@ __global__ __launch_bounds__(64 * WPB) void buildPruneKernel(BuildArgs a, float 
         for (uint32_t t = lane; t < nv; t += 64) S.keys[t] = a.visited[(size_t)i * kVisitCap + t];
         for (uint32_t t = lane; t < dp; t += 64) S.rowT[t] = a.adj[(size_t)p * a.R + t];
         __builtin_amdgcn_wave_barrier();
-        keysFromIds(a, p, S.rowT, dp, S.keys + nv, lane);
-        const uint32_t outc = robustPruneWave<CAP>(a, p, alpha, S, nc, lane);
+        keysFromIds(a.vec, a.normsInRow, p, S.rowT, dp, S.keys + nv, lane);
+        const uint32_t outc = robustPruneWave<CAP, false>(a.vec, a.normsInRow, a.R, p, alpha, S, nc, lane);
         for (uint32_t t = lane; t < a.R; t += 64) {
             const uint32_t x = t < outc ? S.out[t] : kEmpty;
             a.adj[(size_t)p * a.R + t] = x;
@@ -506,11 +555,227 @@ __global__ __launch_bounds__(64 * WPB) void buildReverseKernel(BuildArgs a, floa
             }
             continue;
         }
-        keysFromIds(a, t, S.list, nadd, S.keys, lane);
-        keysFromIds(a, t, S.rowT, dt, S.keys + nadd, lane);
-        const uint32_t outc = robustPruneWave<CAP>(a, t, alpha, S, nadd + dt, lane);
+        keysFromIds(a.vec, a.normsInRow, t, S.list, nadd, S.keys, lane);
+        keysFromIds(a.vec, a.normsInRow, t, S.rowT, dt, S.keys + nadd, lane);
+        const uint32_t outc = robustPruneWave<CAP, false>(a.vec, a.normsInRow, a.R, t, alpha, S, nadd + dt, lane);
         for (uint32_t r = lane; r < a.R; r += 64) a.adj[(size_t)t * a.R + r] = r < outc ? S.out[r] : kEmpty;
         if (lane == 0) a.deg[t] = outc;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ---- HNSW ----
+
+struct HnswBuildArgs {
+    const int8_t* vec;
+    int32_t normsInRow;
+    uint32_t n, M, M0, efc, top, layers, entry;
+    uint32_t* adj0;             // n x M0
+    uint32_t* deg0;             // n
+    uint32_t* adjUp;            // blocks x M
+    uint32_t* degUp;            // blocks
+    const uint32_t* upStart;    // n: first upper-layer block of each node
+    const uint32_t* level;      // n
+    const uint32_t* blockNode;  // blocks: the node owning each upper-layer block
+    const uint32_t* batch;
+    uint32_t size;
+    uint64_t* W;                // size x layers x efc: each layer's final frontier (keys)
+    uint32_t* Wn;               // size x layers: its length; kEmpty where not searched
+    uint64_t* wvis;             // per search wavefront: kVisitCap expanded keys
+    uint32_t* marks;
+    uint32_t markWords;
+    uint32_t searchWaves;
+    uint64_t* pairs;            // size x pairStride new edges (row id << 32 | source), ~0 unused
+    uint32_t pairStride;        // M0 + top M
+    const uint64_t* sorted;
+    uint32_t npairs;
+    const uint32_t* heads;
+    const uint32_t* nheads;
+    uint32_t* deferred;
+    uint32_t* ndeferred;
+    uint32_t* err;
+};
+
+__device__ __forceinline__ LayerRows hnswRows(const HnswBuildArgs& a, uint32_t l) {
+    return l == 0 ? LayerRows{a.adj0, a.deg0, a.M0, nullptr, 0} : LayerRows{a.adjUp, a.degUp, a.M, a.upStart, l};
+}
+
+// Row id of node t's list on layer l: t on layer 0, n + its block above.
+__device__ __forceinline__ uint32_t hnswRowId(const HnswBuildArgs& a, uint32_t t, uint32_t l) {
+    return l == 0 ? t : a.n + a.upStart[t] + l - 1;
+}
+
+// Each batch point: greedy descent (width 1) from the entry down to its level, then a
+// width-efc search on every layer from there to 0, each layer starting from the previous
+// layer's final frontier; the frontiers are the selection's input.
+template <int C>
+__global__ __launch_bounds__(64 * kSearchWaves) void hnswBuildSearchKernel(HnswBuildArgs a) {
+    __shared__ uint64_t mkey[kSearchWaves][64 * C];
+    __shared__ unsigned char mflag[kSearchWaves][64 * C];
+    __shared__ uint32_t pend[kSearchWaves][128];
+    __shared__ uint32_t eps[kSearchWaves][64 * C];
+    const int lane = laneId(), w = threadIdx.x >> 6;
+    const uint32_t wave = blockIdx.x * kSearchWaves + w;
+    uint32_t* marks = a.marks + (size_t)wave * a.markWords;
+    uint64_t* vis = a.wvis + (size_t)wave * kVisitCap;
+    const int sub = lane & 7;
+    const int normMask = (a.normsInRow && sub == 7) ? -1 : 0;
+    for (uint32_t i = wave; i < a.size; i += a.searchWaves) {
+        const uint32_t p = a.batch[i];
+        const uint32_t lp = min(a.level[p], a.top);
+        int4 qv = *reinterpret_cast<const int4*>(a.vec + (size_t)p * kVecBytes + sub * 16);
+        qv.w &= ~normMask;
+        const int qq = groupSum8(sliceNorm(qv));
+        if (lane == 0) eps[w][0] = a.entry;
+        __builtin_amdgcn_wave_barrier();
+        uint32_t neps = 1;
+        bool bad = false;
+        uint64_t fk[C];
+        for (uint32_t l = a.top; l > lp && !bad; l--) {
+            uint32_t nv = 0;
+            exactSearch<C>(a.vec, hnswRows(a, l), eps[w], neps, 1, qv, qq, normMask, marks, mkey[w], mflag[w],
+                           pend[w], vis, kVisitCap, fk, &nv, lane);
+            if (nv > kVisitCap) {
+                bad = true;
+                break;
+            }
+            const uint32_t best = (uint32_t)readlane64(fk[0], 0);
+            if (lane == 0) eps[w][0] = best;
+            __builtin_amdgcn_wave_barrier();
+            neps = 1;
+        }
+        for (int l = (int)lp; l >= 0 && !bad; l--) {
+            uint32_t nv = 0;
+            const uint32_t F = exactSearch<C>(a.vec, hnswRows(a, (uint32_t)l), eps[w], neps, a.efc, qv, qq, normMask,
+                                              marks, mkey[w], mflag[w], pend[w], vis, kVisitCap, fk, &nv, lane);
+            if (nv > kVisitCap) {
+                bad = true;
+                break;
+            }
+            uint64_t* Wl = a.W + ((size_t)i * a.layers + l) * a.efc;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const uint32_t idx = c * 64 + lane;
+                if (idx < F) {
+                    Wl[idx] = fk[c];
+                    eps[w][idx] = (uint32_t)fk[c];
+                }
+            }
+            if (lane == 0) a.Wn[(size_t)i * a.layers + l] = F;
+            __builtin_amdgcn_wave_barrier();
+            neps = F;
+        }
+        if (bad) {
+            for (uint32_t l = lane; l < a.layers; l += 64) a.Wn[(size_t)i * a.layers + l] = kEmpty;
+            if (lane == 0) atomicOr(a.err, kErrVisit);
+        }
+    }
+}
+
+// Each batch point and each of its layers: the frontier selected with the heuristic to
+// M0 (layer 0) or M neighbours, written as the point's list; every pair slot is written.
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void hnswBuildSelectKernel(HnswBuildArgs a, float alpha) {
+    __shared__ PruneLds<kSmallCap> lds[WPB];
+    const int lane = laneId(), w = threadIdx.x >> 6;
+    PruneLds<kSmallCap>& S = lds[w];
+    for (uint32_t i = blockIdx.x * WPB + w; i < a.size; i += gridDim.x * WPB) {
+        const uint32_t p = a.batch[i];
+        const uint32_t lp = min(a.level[p], a.top);
+        uint64_t* pr = a.pairs + (size_t)i * a.pairStride;
+        for (uint32_t l = 0; l <= a.top; l++) {
+            const uint32_t m = l == 0 ? a.M0 : a.M;
+            const uint32_t off = l == 0 ? 0 : a.M0 + (l - 1) * a.M;
+            const uint32_t nc = l <= lp ? a.Wn[(size_t)i * a.layers + l] : kEmpty;
+            if (nc == kEmpty || nc > a.efc) {
+                for (uint32_t t = lane; t < m; t += 64) pr[off + t] = ~0ull;
+                continue;
+            }
+            const uint64_t* Wl = a.W + ((size_t)i * a.layers + l) * a.efc;
+            for (uint32_t t = lane; t < nc; t += 64) S.keys[t] = Wl[t];
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t outc = robustPruneWave<kSmallCap, true>(a.vec, a.normsInRow, m, p, alpha, S, nc, lane);
+            uint32_t* row = l == 0 ? a.adj0 + (size_t)p * a.M0 : a.adjUp + ((size_t)a.upStart[p] + l - 1) * a.M;
+            for (uint32_t t = lane; t < m; t += 64) {
+                const uint32_t x = t < outc ? S.out[t] : kEmpty;
+                row[t] = x;
+                pr[off + t] = t < outc ? (((uint64_t)hnswRowId(a, x, l) << 32) | p) : ~0ull;
+            }
+            if (lane == 0) {
+                if (l == 0) a.deg0[p] = outc;
+                else a.degUp[a.upStart[p] + l - 1] = outc;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
+// Each (layer, target) with new in-edges: absent sources appended in source order, or,
+// past the layer's width, the list and those sources selected together.  items == nullptr:
+// every run, runs wider than CAP deferred; else the deferred head indices.
+template <uint32_t CAP, int WPB>
+__global__ __launch_bounds__(64 * WPB) void hnswBuildReverseKernel(HnswBuildArgs a, float alpha, const uint32_t* items) {
+    __shared__ PruneLds<CAP> lds[WPB];
+    const int lane = laneId(), w = threadIdx.x >> 6;
+    PruneLds<CAP>& S = lds[w];
+    const uint32_t count = items ? *a.ndeferred : *a.nheads;
+    for (uint32_t it = blockIdx.x * WPB + w; it < count; it += gridDim.x * WPB) {
+        const uint32_t h = items ? items[it] : it;
+        const uint32_t start = a.heads[h];
+        const uint32_t rid = (uint32_t)(a.sorted[start] >> 32);
+        uint32_t t, cap;
+        uint32_t* row;
+        uint32_t* degp;
+        if (rid < a.n) {
+            t = rid;
+            cap = a.M0;
+            row = a.adj0 + (size_t)t * a.M0;
+            degp = a.deg0 + t;
+        } else {
+            const uint32_t b = rid - a.n;
+            t = a.blockNode[b];
+            cap = a.M;
+            row = a.adjUp + (size_t)b * a.M;
+            degp = a.degUp + b;
+        }
+        const uint32_t dt = *degp;
+        for (uint32_t r = lane; r < dt; r += 64) S.rowT[r] = row[r];
+        __builtin_amdgcn_wave_barrier();
+        uint32_t nadd = 0;
+        for (uint32_t b = start;; b += 64) {
+            const uint32_t idx = b + lane;
+            const uint64_t x = idx < a.npairs ? a.sorted[idx] : ~0ull;
+            const bool inRun = x != ~0ull && (uint32_t)(x >> 32) == rid;
+            const uint32_t s = (uint32_t)x;
+            bool present = false;
+            if (inRun)
+                for (uint32_t r = 0; r < dt; r++) present |= S.rowT[r] == s;
+            const bool keep = inRun && !present && s != t;
+            const uint64_t km = ballot(keep);
+            const uint32_t pos = nadd + mbcnt(km);
+            if (keep && pos < CAP) S.list[pos] = s;
+            nadd += (uint32_t)__popcll(km);
+            if (ballot(inRun) != ~0ull) break;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (nadd == 0) continue;
+        if (dt + nadd <= cap) {
+            for (uint32_t r = lane; r < nadd; r += 64) row[dt + r] = S.list[r];
+            if (lane == 0) *degp = dt + nadd;
+            continue;
+        }
+        if (nadd + dt > CAP) {
+            if (lane == 0) {
+                if (items == nullptr) a.deferred[atomicAdd(a.ndeferred, 1u)] = h;
+                else atomicOr(a.err, kErrPrune);
+            }
+            continue;
+        }
+        keysFromIds(a.vec, a.normsInRow, t, S.rowT, dt, S.keys, lane);
+        keysFromIds(a.vec, a.normsInRow, t, S.list, nadd, S.keys + dt, lane);
+        const uint32_t outc = robustPruneWave<CAP, true>(a.vec, a.normsInRow, cap, t, alpha, S, nadd + dt, lane);
+        for (uint32_t r = lane; r < cap; r += 64) row[r] = r < outc ? S.out[r] : kEmpty;
+        if (lane == 0) *degp = outc;
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -652,6 +917,155 @@ hipError_t launchBuildVamana(const int8_t* vec, bool normsInRow, uint32_t n, uin
     }
     BUILD_TRY(hipMemcpyAsync(outDeg, deg, (size_t)n * 4, hipMemcpyDeviceToHost, s));
     BUILD_TRY(hipMemcpyAsync(outAdj, adj, (size_t)n * R * 4, hipMemcpyDeviceToHost, s));
+    BUILD_TRY(hipMemcpyAsync(status, err, 4, hipMemcpyDeviceToHost, s));
+    return hipStreamSynchronize(s);
+}
+
+
+hipError_t launchBuildHnsw(const int8_t* vec, bool normsInRow, uint32_t n, uint32_t M, uint32_t efc, float ml,
+                           float alpha, uint64_t seed, int cus, HnswBuildOut* out, uint32_t* status,
+                           hipStream_t s) {
+    *status = 0;
+    tools::HnswPlan plan = tools::hnswPlan(n, ml, seed);
+    const uint32_t M0 = 2 * M;
+    const uint64_t blocks = plan.blocks;
+    if ((uint64_t)n + blocks >= 0xFFFFFFFFull) return hipErrorInvalidValue;
+    const uint32_t layers = plan.top + 1;
+    const uint32_t pairStride = M0 + plan.top * M;
+    const uint32_t markWords = (n + 31) / 32;
+    const int C = efc <= 64 ? 1 : efc <= 128 ? 2 : 4;
+    uint32_t searchBlocksMax = (uint32_t)std::max(1, cus) * 6;
+    while (searchBlocksMax > 1 && (size_t)searchBlocksMax * kSearchWaves * markWords * 4 > kMarkBudget)
+        searchBlocksMax /= 2;
+    const uint32_t searchWavesMax = searchBlocksMax * kSearchWaves;
+    const uint32_t pruneBlocks = (uint32_t)std::max(1, cus) * 5;
+    const uint32_t wideBlocks = (uint32_t)std::max(1, cus);
+    const uint32_t maxBatch = plan.maxBatch;
+
+    std::vector<uint32_t> up32(n), blockNode(std::max<uint64_t>(1, blocks));
+    for (uint32_t i = 0; i < n; i++) {
+        up32[i] = (uint32_t)plan.upStart[i];
+        for (uint32_t l = 0; l < plan.level[i]; l++) blockNode[plan.upStart[i] + l] = i;
+    }
+
+    DevMem m;
+    uint32_t *adj0, *deg0, *adjUp, *degUp, *dUp, *dLevel, *dBlockNode, *dOrder, *Wn, *marks, *heads, *nheads,
+        *deferred, *ndeferred, *err;
+    uint64_t *W, *wvis, *pairs, *sorted;
+    const size_t nBlocks = std::max<uint64_t>(1, blocks);
+    BUILD_TRY(m.alloc(&adj0, (size_t)n * M0));
+    BUILD_TRY(m.alloc(&deg0, n));
+    BUILD_TRY(m.alloc(&adjUp, nBlocks * M));
+    BUILD_TRY(m.alloc(&degUp, nBlocks));
+    BUILD_TRY(m.alloc(&dUp, n));
+    BUILD_TRY(m.alloc(&dLevel, n));
+    BUILD_TRY(m.alloc(&dBlockNode, nBlocks));
+    BUILD_TRY(m.alloc(&dOrder, std::max<size_t>(1, plan.order.size())));
+    BUILD_TRY(m.alloc(&W, (size_t)maxBatch * layers * efc));
+    BUILD_TRY(m.alloc(&Wn, (size_t)maxBatch * layers));
+    BUILD_TRY(m.alloc(&wvis, (size_t)searchWavesMax * kVisitCap));
+    BUILD_TRY(m.alloc(&marks, (size_t)searchWavesMax * markWords));
+    BUILD_TRY(m.alloc(&pairs, (size_t)maxBatch * pairStride));
+    BUILD_TRY(m.alloc(&sorted, (size_t)maxBatch * pairStride));
+    BUILD_TRY(m.alloc(&heads, (size_t)maxBatch * pairStride));
+    BUILD_TRY(m.alloc(&deferred, (size_t)maxBatch * pairStride));
+    BUILD_TRY(m.alloc(&nheads, 1));
+    BUILD_TRY(m.alloc(&ndeferred, 1));
+    BUILD_TRY(m.alloc(&err, 1));
+    size_t tempBytes = 0;
+    BUILD_TRY(rocprim::radix_sort_keys(nullptr, tempBytes, pairs, sorted, (size_t)maxBatch * pairStride, 0, 64, s));
+    unsigned char* temp;
+    BUILD_TRY(m.alloc(&temp, tempBytes));
+    BUILD_TRY(hipMemsetAsync(adj0, 0xFF, (size_t)n * M0 * 4, s));
+    BUILD_TRY(hipMemsetAsync(deg0, 0, (size_t)n * 4, s));
+    BUILD_TRY(hipMemsetAsync(adjUp, 0xFF, nBlocks * M * 4, s));
+    BUILD_TRY(hipMemsetAsync(degUp, 0, nBlocks * 4, s));
+    BUILD_TRY(hipMemsetAsync(marks, 0, (size_t)searchWavesMax * markWords * 4, s));
+    BUILD_TRY(hipMemsetAsync(err, 0, 4, s));
+    BUILD_TRY(hipMemcpyAsync(dUp, up32.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+    BUILD_TRY(hipMemcpyAsync(dLevel, plan.level.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+    BUILD_TRY(hipMemcpyAsync(dBlockNode, blockNode.data(), nBlocks * 4, hipMemcpyHostToDevice, s));
+    if (!plan.order.empty())
+        BUILD_TRY(hipMemcpyAsync(dOrder, plan.order.data(), plan.order.size() * 4, hipMemcpyHostToDevice, s));
+
+    HnswBuildArgs a = {};
+    a.vec = vec;
+    a.normsInRow = normsInRow ? 1 : 0;
+    a.n = n;
+    a.M = M;
+    a.M0 = M0;
+    a.efc = efc;
+    a.top = plan.top;
+    a.layers = layers;
+    a.entry = plan.entry;
+    a.adj0 = adj0;
+    a.deg0 = deg0;
+    a.adjUp = adjUp;
+    a.degUp = degUp;
+    a.upStart = dUp;
+    a.level = dLevel;
+    a.blockNode = dBlockNode;
+    a.W = W;
+    a.Wn = Wn;
+    a.wvis = wvis;
+    a.marks = marks;
+    a.markWords = markWords;
+    a.pairs = pairs;
+    a.pairStride = pairStride;
+    a.sorted = sorted;
+    a.heads = heads;
+    a.nheads = nheads;
+    a.deferred = deferred;
+    a.ndeferred = ndeferred;
+    a.err = err;
+
+    const size_t total = plan.order.size();
+    size_t count = 0;
+    uint32_t bsize = 1;
+    while (count < total) {
+        const uint32_t size = (uint32_t)std::min<size_t>(bsize, total - count);
+        a.batch = dOrder + count;
+        a.size = size;
+        count += size;
+        bsize = std::min(maxBatch, bsize * 2);
+        const uint32_t blocksS = std::min(searchBlocksMax, (size + kSearchWaves - 1) / kSearchWaves);
+        a.searchWaves = blocksS * kSearchWaves;
+        switch (C) {
+            case 1: hipLaunchKernelGGL((hnswBuildSearchKernel<1>), dim3(blocksS), dim3(64 * kSearchWaves), 0, s, a); break;
+            case 2: hipLaunchKernelGGL((hnswBuildSearchKernel<2>), dim3(blocksS), dim3(64 * kSearchWaves), 0, s, a); break;
+            default: hipLaunchKernelGGL((hnswBuildSearchKernel<4>), dim3(blocksS), dim3(64 * kSearchWaves), 0, s, a); break;
+        }
+        BUILD_TRY(hipGetLastError());
+        const uint32_t pb = std::min(pruneBlocks, (size + kPruneWaves - 1) / kPruneWaves);
+        hipLaunchKernelGGL((hnswBuildSelectKernel<kPruneWaves>), dim3(pb), dim3(64 * kPruneWaves), 0, s, a, alpha);
+        BUILD_TRY(hipGetLastError());
+        a.npairs = size * pairStride;
+        size_t tb = tempBytes;
+        BUILD_TRY(rocprim::radix_sort_keys(temp, tb, pairs, sorted, (size_t)a.npairs, 0, 64, s));
+        BUILD_TRY(hipMemsetAsync(nheads, 0, 4, s));
+        BUILD_TRY(hipMemsetAsync(ndeferred, 0, 4, s));
+        hipLaunchKernelGGL(buildHeadsKernel, dim3((a.npairs + 255) / 256), dim3(256), 0, s, sorted, a.npairs,
+                           heads, nheads);
+        BUILD_TRY(hipGetLastError());
+        hipLaunchKernelGGL((hnswBuildReverseKernel<kSmallCap, kPruneWaves>), dim3(pruneBlocks),
+                           dim3(64 * kPruneWaves), 0, s, a, alpha, nullptr);
+        BUILD_TRY(hipGetLastError());
+        hipLaunchKernelGGL((hnswBuildReverseKernel<kSortCap, 1>), dim3(wideBlocks), dim3(64), 0, s, a, alpha,
+                           static_cast<const uint32_t*>(deferred));
+        BUILD_TRY(hipGetLastError());
+    }
+    out->level = std::move(plan.level);
+    out->upStart = std::move(plan.upStart);
+    out->entry = plan.entry;
+    out->M0 = M0;
+    out->deg0.resize(n);
+    out->adj0.resize((size_t)n * M0);
+    out->degUp.resize(nBlocks);
+    out->adjUp.resize(nBlocks * M);
+    BUILD_TRY(hipMemcpyAsync(out->deg0.data(), deg0, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    BUILD_TRY(hipMemcpyAsync(out->adj0.data(), adj0, (size_t)n * M0 * 4, hipMemcpyDeviceToHost, s));
+    BUILD_TRY(hipMemcpyAsync(out->degUp.data(), degUp, nBlocks * 4, hipMemcpyDeviceToHost, s));
+    BUILD_TRY(hipMemcpyAsync(out->adjUp.data(), adjUp, nBlocks * M * 4, hipMemcpyDeviceToHost, s));
     BUILD_TRY(hipMemcpyAsync(status, err, 4, hipMemcpyDeviceToHost, s));
     return hipStreamSynchronize(s);
 }

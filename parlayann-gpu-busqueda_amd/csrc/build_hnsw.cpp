@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/pann_tools.h"
+#include "hnsw_plan.h"
 #include "tools_common.h"
 #include <cstdlib>
 #include <chrono>
@@ -150,31 +151,12 @@ struct Hnsw {
         }, 4);
     }
 
-    int write(const char* path, float ml) const {
-        FILE* f = fopen(path, "wb");
-        if (!f) return setError(2, std::string("cannot write ") + path);
-        bool ok = fwrite("HNSW", 1, 4, f) == 4;
-        const uint32_t version = 3;
-        const uint64_t zero = 0;
-        ok = ok && fwrite(&version, 4, 1, f) == 1 && fwrite(&zero, 8, 1, f) == 1 &&
-             fwrite(&zero, 8, 1, f) == 1;
-        ok = ok && fwrite(&dim, 4, 1, f) == 1 && fwrite(&ml, 4, 1, f) == 1 &&
-             fwrite(&M, 4, 1, f) == 1 && fwrite(&efc, 4, 1, f) == 1 &&
-             fwrite(&alpha, 4, 1, f) == 1 && fwrite(&n, 4, 1, f) == 1;
-        for (uint32_t i = 0; ok && i < n; i++) {
-            const uint32_t rec[2] = {level[i], i};
-            ok = fwrite(rec, 4, 2, f) == 2;
-        }
-        for (uint32_t i = 0; ok && i < n; i++)
-            for (uint32_t l = 0; ok && l <= level[i]; l++) {
-                const uint64_t c = const_cast<Hnsw*>(this)->deg(i, l);
-                ok = fwrite(&c, 8, 1, f) == 1 &&
-                     fwrite(const_cast<Hnsw*>(this)->row(i, l), 4, c, f) == c;
-            }
-        const uint64_t ne = 1;
-        ok = ok && fwrite(&ne, 8, 1, f) == 1 && fwrite(&entry, 4, 1, f) == 1;
-        ok = (fclose(f) == 0) && ok;
-        if (!ok) return setError(2, std::string("short write to ") + path);
+    int write(const char* path, float ml) {
+        const int r = writeHnswV3(path, n, dim, ml, M, efc, alpha, level,
+                                  [&](uint32_t i, uint32_t l) { return deg(i, l); },
+                                  [&](uint32_t i, uint32_t l) { return row(i, l); }, entry);
+        if (r == 1) return setError(2, std::string("cannot write ") + path);
+        if (r != 0) return setError(2, std::string("short write to ") + path);
         return 0;
     }
 };
@@ -198,32 +180,21 @@ extern "C" int pann_build_hnsw(const int8_t* base, uint32_t n, uint32_t dim, uin
     h.efc = efc;
     h.alpha = alpha > 0 ? alpha : 1.0f;
     h.nthreads = std::max(1, nthreads);
-    h.level.resize(n);
-    Rng lr(mixSeed(seed, 2));
-    for (uint32_t i = 0; i < n; i++) {
-        double u = 1.0 - lr.uniform();  // (0, 1]
-        h.level[i] = (uint32_t)std::min(30.0, std::floor(-std::log(u) * ml));
-    }
-    h.upStart.assign(n, 0);
-    uint64_t blocks = 0;
-    for (uint32_t i = 0; i < n; i++) {
-        h.upStart[i] = blocks;
-        blocks += h.level[i];
-    }
+    HnswPlan plan = hnswPlan(n, ml, seed);
+    h.level = std::move(plan.level);
+    h.upStart = std::move(plan.upStart);
+    const uint64_t blocks = plan.blocks;
     h.adj0.assign((size_t)n * h.M0, 0xFFFFFFFFu);
     h.deg0.assign(n, 0);
     h.adjUp.assign(std::max<uint64_t>(1, blocks) * M, 0xFFFFFFFFu);
     h.degUp.assign(std::max<uint64_t>(1, blocks), 0);
-    h.entry = 0;
-    for (uint32_t i = 1; i < n; i++)
-        if (h.level[i] > h.level[h.entry]) h.entry = i;
-    h.top = h.level[h.entry];
+    h.entry = plan.entry;
+    h.top = plan.top;
 
-    std::vector<uint32_t> order = permutation(n, mixSeed(seed, 3));
-    order.erase(std::find(order.begin(), order.end(), h.entry));
+    const std::vector<uint32_t>& order = plan.order;
     std::vector<BuildSearcher> searchers;
     for (int t = 0; t < h.nthreads; t++) searchers.emplace_back(n);
-    const uint32_t maxBatch = std::max<uint32_t>(1, (uint32_t)(0.02 * n));
+    const uint32_t maxBatch = plan.maxBatch;
     size_t count = 0;
     uint32_t bsize = 1;
     // PANN_BUILD_PROGRESS=1: a line on stderr at every tenth of the points inserted

@@ -89,6 +89,7 @@ struct Vamana {
                 const uint32_t p = batch[i];
                 deg[p] = (uint32_t)newOut[i].size();
                 std::copy(newOut[i].begin(), newOut[i].end(), adj.begin() + (size_t)p * R);
+                std::fill(adj.begin() + (size_t)p * R + deg[p], adj.begin() + (size_t)(p + 1) * R, 0xFFFFFFFFu);
             }
             // 3. reverse edges, grouped by target
             std::vector<std::pair<uint32_t, uint32_t>> rev;
@@ -119,6 +120,7 @@ struct Vamana {
                     robustPrune(t, cand, alpha, out, true);
                     deg[t] = (uint32_t)out.size();
                     std::copy(out.begin(), out.end(), adj.begin() + (size_t)t * R);
+                    std::fill(adj.begin() + (size_t)t * R + deg[t], adj.begin() + (size_t)(t + 1) * R, 0xFFFFFFFFu);
                 }
             }, 4);
         }

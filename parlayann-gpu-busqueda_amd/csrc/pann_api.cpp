@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "formats.h"
+#include "hnsw_plan.h"
 #include "pann_objects.h"
 
 using namespace pann;
@@ -805,6 +806,37 @@ int pann_build_vamana_gpu(const pann_vectors* base, uint32_t R, uint32_t L, floa
     if (e != hipSuccess) return setError(PANN_ERR_HIP, "Vamana build: %s", hipGetErrorString(e));
     if (status & 1u) return setError(PANN_ERR_CAPACITY, "a build search expanded more than 1024 nodes");
     if (status & 2u) return setError(PANN_ERR_CAPACITY, "a robust prune met more than 2048 candidates");
+    return PANN_OK;
+}
+
+int pann_build_hnsw_gpu(const pann_vectors* base, uint32_t M, uint32_t efc, float ml, float alpha,
+                        uint64_t seed, const char* out_path, void* stream) {
+    if (!base || !out_path) return setError(PANN_ERR_ARG, "NULL argument");
+    if (base->n == 0) return setError(PANN_ERR_ARG, "no points");
+    if (M == 0 || M > 64) return setError(PANN_ERR_CAPACITY, "M=%u outside 1..64", M);
+    if (efc == 0 || efc > 256) return setError(PANN_ERR_CAPACITY, "efc=%u outside 1..256", efc);
+    if (!(ml > 0)) return setError(PANN_ERR_ARG, "ml must be positive");
+    const float al = alpha > 0 ? alpha : 1.0f;
+    int cus = 0;
+    int r = deviceCUs(&cus);
+    if (r != PANN_OK) return r;
+    HnswBuildOut h;
+    uint32_t status = 0;
+    const hipError_t e = launchBuildHnsw(base->dev, base->dim <= (uint32_t)kNormInRowMaxDim, base->n, M, efc, ml,
+                                         al, seed, cus, &h, &status, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return setError(PANN_ERR_CAPACITY, "too many upper-layer lists");
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "HNSW build: %s", hipGetErrorString(e));
+    if (status & 1u) return setError(PANN_ERR_CAPACITY, "a build search expanded more than 1024 nodes");
+    if (status & 2u) return setError(PANN_ERR_CAPACITY, "a neighbour selection met more than 2048 candidates");
+    const int w = tools::writeHnswV3(
+        out_path, base->n, base->dim, ml, M, efc, al, h.level,
+        [&](uint32_t i, uint32_t l) { return l == 0 ? h.deg0[i] : h.degUp[h.upStart[i] + l - 1]; },
+        [&](uint32_t i, uint32_t l) {
+            return l == 0 ? h.adj0.data() + (size_t)i * h.M0 : h.adjUp.data() + (h.upStart[i] + l - 1) * M;
+        },
+        h.entry);
+    if (w == 1) return setError(PANN_ERR_IO, "cannot write %s", out_path);
+    if (w != 0) return setError(PANN_ERR_IO, "short write to %s", out_path);
     return PANN_OK;
 }
 

@@ -5,6 +5,8 @@
 #include <stdint.h>
 #include <stdlib.h>
 
+#include <vector>
+
 namespace pann {
 
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // padding of adjacency rows / missing neighbour
@@ -270,6 +272,17 @@ hipError_t launchGatherBench(const int8_t* rows, uint32_t nrows, const int8_t* q
 // dst row i = src row idx[i] (kVecBytes rows); indices >= nsrc leave the row untouched.
 hipError_t launchGatherRows(int8_t* dst, const int8_t* src, const uint32_t* idx, uint32_t n,
                             uint32_t nsrc, hipStream_t s);
+// HNSW graph construction (build_gpu.hip): node levels, layer-0 lists (n x M0) and upper
+// lists (one M-wide block per node and layer above 0, from upStart) on the host.
+struct HnswBuildOut {
+    std::vector<uint32_t> level;
+    std::vector<uint64_t> upStart;
+    uint32_t entry = 0, M0 = 0;
+    std::vector<uint32_t> deg0, adj0, degUp, adjUp;
+};
+hipError_t launchBuildHnsw(const int8_t* vec, bool normsInRow, uint32_t n, uint32_t M, uint32_t efc, float ml,
+                           float alpha, uint64_t seed, int cus, HnswBuildOut* out, uint32_t* status,
+                           hipStream_t s);
 // Vamana graph construction (build_gpu.hip) into host arrays; *status 1: a search
 // expanded too many nodes, 2: a prune met too many candidates.
 hipError_t launchBuildVamana(const int8_t* vec, bool normsInRow, uint32_t n, uint32_t R, uint32_t L,

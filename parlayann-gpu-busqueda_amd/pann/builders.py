@@ -70,6 +70,24 @@ def build_hnsw(base, M=32, efc=128, ml=None, alpha=1.0, seed=7, nthreads=None, p
         os.unlink(tmp)
 
 
+def build_hnsw_gpu(vectors, M=32, efc=128, ml=None, alpha=1.0, seed=7, path=None):
+    """build_hnsw's index, built on the device from the rows of a search.Vectors
+    (pann_build_hnsw_gpu); its v3 file equals build_hnsw's for the same arguments."""
+    from ._lib import check, lib
+    ml = float(ml if ml is not None else 1.0 / np.log(M))
+
+    def run(p):
+        check(lib().pann_build_hnsw_gpu(vectors.handle, M, efc, ml, float(alpha), seed, p, None))
+    if path:
+        run(str(path).encode())
+        return formats.read_hnsw(path)
+    tmp = _via_file(run, ".hnsw")
+    try:
+        return formats.read_hnsw(tmp)
+    finally:
+        os.unlink(tmp)
+
+
 def build_hcnng(base, num_trees=30, leaf_size=1000, mst_deg=3, seed=7, nthreads=None, path=None):
     base = np.ascontiguousarray(base, dtype=np.int8)
     n, dim = base.shape

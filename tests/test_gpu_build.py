@@ -53,3 +53,29 @@ def test_gpu_build_rejects_bad_arguments(gpu):
             builders.build_vamana_gpu(b, R=8, L=257)
     finally:
         b.free()
+
+
+HNSW_CASES = [
+    # n, dim, M, efc, alpha, seed
+    (1, 16, 4, 8, 1.0, 1),
+    (400, 16, 4, 16, 1.0, 3),
+    (4000, 100, 16, 64, 1.0, 7),
+    (4000, 128, 8, 100, 1.2, 5),
+    (6000, 37, 32, 200, 1.0, 11),
+]
+
+
+@pytest.mark.parametrize("n,dim,M,efc,alpha,seed", HNSW_CASES)
+def test_gpu_hnsw_build_equals_host(gpu, tmp_path, n, dim, M, efc, alpha, seed):
+    """The HNSW v3 file written by the device build equals the host builder's, byte for
+    byte (levels, every layer's lists, entry point); ml = 1/ln M."""
+    from pann import builders, search
+    base = _data(n, dim, seed)
+    print(f"[build] hnsw n={n} dim={dim} M={M} efc={efc} alpha={alpha}", flush=True)
+    builders.build_hnsw(base, M=M, efc=efc, alpha=alpha, seed=seed, nthreads=8, path=tmp_path / "host.hnsw")
+    b = search.Vectors(base)
+    try:
+        builders.build_hnsw_gpu(b, M=M, efc=efc, alpha=alpha, seed=seed, path=tmp_path / "gpu.hnsw")
+    finally:
+        b.free()
+    assert (tmp_path / "gpu.hnsw").read_bytes() == (tmp_path / "host.hnsw").read_bytes()
