@@ -317,8 +317,9 @@ def batched_vs_plain(b, g, rows, params, steps, tune_sample=8192):
     t = time.time()
     cfg, tune_ms = search.autotune(rows[:tune_sample], params[:tune_sample], max_Ls, max_k,
                                    base=b, graph=g)
-    log(f"[bench] autotune {time.time() - t:.1f}s -> waves/block {cfg.waves_per_block}, LDS filter "
-        f"bits {cfg.lds_filter_bits}, visited slots {cfg.visited_slots}")
+    log(f"[bench] autotune {time.time() - t:.1f}s -> waves/block {cfg.waves_per_block}, waves/query "
+        f"{max(1, cfg.waves_per_query)}, LDS filter bits {cfg.lds_filter_bits}, visited slots "
+        f"{cfg.visited_slots}, prefetch {cfg.prefetch}")
     st = search.Stream()
     batch = search.Batch(nq, max_Ls, max_k, base=b, graph=g, config=cfg)
     batch.set_queries(rows, params, st.ptr)
@@ -401,8 +402,9 @@ def batched_vs_plain(b, g, rows, params, steps, tune_sample=8192):
         "qps_plain_4_streams": round(nq / (plain4_ms / 1e3), 1),
         "ratio_vs_plain_4_streams": round(plain4_ms / batched_ms, 2),
         "plain_4_streams_ms": round(plain4_ms, 3),
-        "autotuned": {"waves_per_block": cfg.waves_per_block, "lds_filter_bits": cfg.lds_filter_bits,
-                      "visited_slots": cfg.visited_slots, "tune_ms": round(tune_ms, 3)},
+        "autotuned": {"waves_per_block": cfg.waves_per_block, "waves_per_query": max(1, cfg.waves_per_query),
+                      "lds_filter_bits": cfg.lds_filter_bits, "visited_slots": cfg.visited_slots,
+                      "prefetch": cfg.prefetch, "tune_ms": round(tune_ms, 3)},
         "queries_differing_from_plain": mismatches, "overflow_or_config_flags": flags,
     }
 
@@ -818,6 +820,7 @@ def main():
             "batched_shape": None if bcfg is None else {
                 "waves_per_block": bcfg["waves_per_block"], "lds_filter_bits": bcfg["lds_filter_bits"],
                 "visited_slots": bcfg["visited_slots"], "prefetch": bcfg["prefetch"],
+                "waves_per_query": max(1, bcfg.get("waves_per_query", 1)),
                 "resident_waves": bwaves, "autotuned": not args.no_tune, "autotune_s": round(btune, 2)},
             "kernel_ms": round(ev_ms, 4), "comps_per_query": round(float(r.comps.mean()), 1),
             "edges_per_query": round(float(r.edges.mean()), 1),

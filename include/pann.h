@@ -212,6 +212,11 @@ typedef struct pann_batch_config {
                                 4096 slots), power of two >= 64 (0: derived) */
     uint32_t prefetch;       /* next-row prefetch depth, 0 or 1: while a hop merges, load the
                                 row of the predicted next node (flat graphs, max_Ls <= 64) */
+    uint32_t waves_per_query; /* wavefronts sharing one query, 1, 2 or 4 (0: 1), in the flat
+                                 launches of the general form (beams above 128 or rows above
+                                 64 entries): the leader filters and merges the candidates,
+                                 all compute distances and move frontier chunks; the outputs
+                                 do not depend on it (one query per block of that many) */
 } pann_batch_config;
 
 /* Allocates device buffers for up to max_queries queries over a flat graph or HNSW. */

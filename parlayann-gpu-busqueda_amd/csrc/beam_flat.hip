@@ -457,6 +457,354 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
 }
 
 // ---------------------------------------------------------------------------------------
+// Cooperative general form: WPQ wavefronts (one block) share one query and its LDS.  The
+// semantics and every step are searchOne's; the work of a hop is split as follows:
+//   * wavefront 0 (the leader) reads the row, updates the visited multiset and runs the
+//     seen filter in row order, publishing the kept ids and the cutoff;
+//   * every wavefront computes the distances (and expansion counts) of the candidates
+//     whose round (candidate / 8) it owns, round r to wavefront r % WPQ, into LDS keys;
+//   * the leader sorts the keys and computes the union marks, publishing the counts;
+//   * every wavefront merges chunks of 64 merged positions, the chunks taken from the
+//     top in rounds of WPQ: a round reads all its sources, then writes (a position's
+//     source never lies above it, so later rounds read positions below every write);
+//   * every wavefront applies the cut and picks the next node itself: same LDS, same
+//     result, no broadcast.
+// Block barriers separate the phases; one query per block keeps them per query.
+template <int MAXCH, bool FGLOBAL, bool VGLOBAL, int WPQ>
+__device__ void searchCoop(const FlatLaunch& L, unsigned char* lds, uint32_t q, const QueryCfg& c,
+                           uint32_t* gfilt, uint32_t* gvk, unsigned char* gvc, int w) {
+    const int lane = laneId();
+    const int grp = lane >> 3, sub = lane & 7;
+    const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const bool leader = w == 0;
+
+    uint64_t* F = reinterpret_cast<uint64_t*>(lds + L.lds.fkey);
+    unsigned char* FV = lds + L.lds.fvis;
+    unsigned char* occ = lds + L.lds.occ;
+    uint64_t* kkey = reinterpret_cast<uint64_t*>(lds + L.lds.kkey);
+    unsigned char* kvis = lds + L.lds.kvis;
+    uint32_t* pend = reinterpret_cast<uint32_t*>(lds + L.lds.pend);
+    unsigned char* tag = lds + L.lds.tag;
+    uint64_t* ck = reinterpret_cast<uint64_t*>(lds + L.lds.ckey);
+    int* scal = reinterpret_cast<int*>(lds + L.lds.cscal);
+    uint32_t* vkey = VGLOBAL ? gvk : reinterpret_cast<uint32_t*>(lds + L.lds.vkey);
+    unsigned char* vcnt = VGLOBAL ? gvc : lds + L.lds.vcnt;
+    uint32_t* filt = FGLOBAL ? gfilt : reinterpret_cast<uint32_t*>(lds + L.lds.filt);
+
+    const int Ls = c.Ls;
+    const int fbits = flatFilterBits(Ls);
+    const uint32_t fmask = (1u << fbits) - 1;
+    uint32_t capV = flatVisitedCap(Ls);
+    if (capV > L.capV) capV = L.capV;
+    const uint32_t logCap = (uint32_t)ceilLog2(capV);
+    const uint32_t vmask = capV - 1;
+
+    const int normMask = (L.normsInRow && sub == 7) ? -1 : 0;
+    int4 qv = *reinterpret_cast<const int4*>(L.queries + (size_t)q * kVecBytes + sub * 16);
+    qv.w &= ~normMask;
+    const int qq = groupSum8(sliceNorm(qv));
+
+    // fresh tables and the start point (node 0), by the leader
+    if (leader) {
+        const uint4 e4 = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+        filtClear<FGLOBAL>(filt, fbits, lane);
+        uint4* v4 = reinterpret_cast<uint4*>(vkey);
+        for (uint32_t i = lane; i < capV / 4; i += 64) v4[i] = e4;
+        for (int i = lane; i < Ls; i += 64) occ[i] = 0;
+        const int d0 = groupDistance(L.base, 0u, sub, qv, qq, normMask);
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+            F[0] = (uint64_t)(uint32_t)d0 << 32;
+            FV[0] = 0;
+        }
+    }
+    __syncthreads();
+
+    int fsize = 1;
+    int u = 0;
+    int64_t visits = 0;
+    int comps = 1, edges = 0, hops = 0, flags = 0;
+    uint32_t distinctVisited = 0;
+    const uint32_t G = L.G;
+    const uint32_t nch = (G + 63) / 64;
+
+    while (u >= 0 && visits < c.limit) {
+        // ---- leader: row, visit, seen filter ----
+        if (leader) {
+            const uint32_t uid = (uint32_t)F[u];
+            uint32_t a[MAXCH];
+            const uint32_t* row = L.adj + (size_t)uid * G;
+#pragma unroll
+            for (int ch = 0; ch < MAXCH; ch++) {
+                const uint32_t j = ch * 64 + lane;
+                a[ch] = (ch < (int)nch && j < G) ? row[j] : kEmpty;
+            }
+            bool overflow = false;
+            if (lane == 0) {
+                FV[u] = 1;
+                overflow = !visitedInsert<VGLOBAL>(vkey, vcnt, uid, logCap, vmask, capV, distinctVisited);
+            }
+            distinctVisited = (uint32_t)readlane((int)distinctVisited, 0);
+            const int ovf = readlane((int)overflow, 0);
+            uint32_t nP = 0;
+            if (!ovf) {
+#pragma unroll
+                for (int ch = 0; ch < MAXCH; ch++) {
+                    if (ch >= (int)nch) break;
+                    const uint32_t ai = a[ch];
+                    edges += __popcll(ballot(ai != kEmpty));
+                    const bool part = (ai != kEmpty) && (ai != c.qid);
+                    const bool keep = seenFilterChunk<FGLOBAL>(filt, tag, fmask, ai, part, lane, lanesBelow);
+                    const uint64_t km = ballot(keep);
+                    if (keep) pend[nP + mbcnt(km)] = ai;
+                    nP += (uint32_t)__popcll(km);
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            if (lane == 0) {
+                scal[0] = (int)nP;
+                scal[1] = (fsize < Ls) ? 0x7FFFFFFF : (int)(F[fsize - 1] >> 32);
+                scal[2] = ovf;
+            }
+        }
+        __syncthreads();
+        const uint32_t nP = (uint32_t)__builtin_amdgcn_readfirstlane(scal[0]);
+        const int cutoff = __builtin_amdgcn_readfirstlane(scal[1]);
+        if (__builtin_amdgcn_readfirstlane(scal[2])) {
+            flags |= kFlagOverflow;
+            break;
+        }
+        visits++;
+        hops++;
+        comps += (int)nP;
+
+        // ---- all: distances of the rounds this wavefront owns ----
+#pragma unroll
+        for (int b = 0; b < MAXCH; b++) {
+            if ((uint32_t)b * 64 >= nP) continue;
+            const uint32_t nb = min(64u, nP - (uint32_t)b * 64);
+            const uint32_t rounds = (nb + 7) / 8;
+            int4 v[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                v[r] = make_int4(0, 0, 0, 0);
+                const uint32_t i = (uint32_t)r * 8 + grp;
+                if (r % WPQ == w && (uint32_t)r < rounds && i < nb) {
+                    const uint32_t id = pend[b * 64 + i];
+                    v[r] = *reinterpret_cast<const int4*>(L.base + (size_t)id * kVecBytes + sub * 16);
+                }
+            }
+            const uint32_t il = (uint32_t)sub * 8 + grp;  // candidate this lane ends up with
+            const bool mine = il < nb && (int)(il >> 3) % WPQ == w;
+            uint32_t cid = 0, cnt = 0;
+            if (mine) {
+                cid = pend[b * 64 + il];
+                cnt = visitedCount<VGLOBAL>(vkey, vcnt, cid, logCap, vmask);
+            }
+            int part8[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                v[r].w &= ~normMask;
+                part8[r] = slicePartial(v[r], qv);
+            }
+            const int tot = transposeReduce8(part8, lane);
+            if (mine) {
+                const int d = qq + tot;
+                ck[b * 64 + il] = d < cutoff ? countKey(d, cid, cnt) : ~0ull;
+            }
+        }
+        __syncthreads();
+
+        // ---- leader: sort, multiset union marks ----
+        if (leader) {
+            uint64_t key[MAXCH];
+#pragma unroll
+            for (int b = 0; b < MAXCH; b++) {
+                const uint32_t i = (uint32_t)b * 64 + lane;
+                key[b] = i < nP ? ck[i] : ~0ull;
+            }
+            bitonicSort<MAXCH>(key, lane);
+            uint64_t runStarts[MAXCH];
+#pragma unroll
+            for (int b = 0; b < MAXCH; b++) {
+                uint64_t up = (uint64_t)__shfl_up((int)(uint32_t)key[b], 1, 64) |
+                              ((uint64_t)(uint32_t)__shfl_up((int)(uint32_t)(key[b] >> 32), 1, 64) << 32);
+                if (lane == 0) up = (b == 0) ? ~key[b] : readlane64(key[b - 1], 63);
+                runStarts[b] = ballot(up != key[b]);
+            }
+            uint32_t kBase = 0;
+            int placed = 0, pmin = -1;
+#pragma unroll
+            for (int b = 0; b < MAXCH; b++) {
+                const bool valid = key[b] != ~0ull;
+                const uint64_t kv = valid ? frontierOfCountKey(key[b]) : ~0ull;
+                const int cnt = (int)(key[b] & 0x1FFull);
+                int rep = 0, lb = 0, cF = 0;
+                if (valid) {
+                    const int e = b * 64 + lane;
+                    uint64_t m = runStarts[b] & (lanesBelow | (1ull << lane));
+                    int start;
+                    if (m) {
+                        start = b * 64 + 63 - __clzll((long long)m);
+                    } else {
+                        start = 0;
+                        for (int bb = b - 1; bb >= 0; bb--)
+                            if (runStarts[bb]) {
+                                start = bb * 64 + 63 - __clzll((long long)runStarts[bb]);
+                                break;
+                            }
+                    }
+                    rep = e - start;
+                    int lo = 0, hi = fsize;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (F[mid] < kv) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    lb = lo;
+                    while (lb + cF < fsize && F[lb + cF] == kv) cF++;
+                }
+                const bool enter = valid && rep >= cF;
+                const uint64_t em = ballot(enter);
+                const uint32_t jK = kBase + mbcnt(em);
+                kBase += (uint32_t)__popcll(em);
+                const int jPos = lb + cF + (int)jK;
+                if (pmin < 0 && em) pmin = readlane(jPos, __ffsll((unsigned long long)em) - 1);
+                placed += __popcll(ballot(enter && jPos < Ls));
+                if (enter) {
+                    kkey[jK] = kv;
+                    kvis[jK] = (unsigned char)(rep < cnt ? 1 : 0);
+                    if (jPos < Ls) occ[jPos] = 1;
+                }
+            }
+            if (lane == 0) {
+                scal[3] = placed;
+                scal[4] = pmin;
+                scal[5] = (int)kBase;
+            }
+        }
+        __syncthreads();
+        const int placed = __builtin_amdgcn_readfirstlane(scal[3]);
+        const int pmin = __builtin_amdgcn_readfirstlane(scal[4]);
+        const int nK = __builtin_amdgcn_readfirstlane(scal[5]);
+
+        // ---- all: merge in place, chunks from the top in rounds of WPQ ----
+        const int uPrev = u;
+        if (placed > 0) {
+            const int newSize = min(Ls, fsize + nK);
+            const int top = pmin + ((newSize - pmin - 1) & ~63);
+            const int nChunks = ((top - pmin) >> 6) + 1;
+            for (int r0 = 0; r0 < nChunks; r0 += WPQ) {
+                const int j = r0 + w;
+                const bool active = j < nChunks;
+                const int p0 = top - 64 * j;
+                const int p = p0 + lane;
+                const bool inb = active && p < newSize;
+                uint64_t nk = 0;
+                unsigned char nv = 0;
+                if (active) {
+                    // entering copies at or above this chunk's end, counted chunk by chunk
+                    int above = 0;
+                    for (int jj = 0; jj < j; jj++) {
+                        const int pp = top - 64 * jj + lane;
+                        above += __popcll(ballot(pp < newSize && occ[pp]));
+                    }
+                    const bool o = inb && occ[p];
+                    const uint64_t ob = ballot(o);
+                    const int rank = (placed - above) - __popcll(ob) + (int)mbcnt(ob);
+                    if (inb) {
+                        if (o) {
+                            nk = kkey[rank];
+                            nv = kvis[rank];
+                        } else {
+                            nk = F[p - rank];
+                            nv = FV[p - rank];
+                        }
+                    }
+                }
+                __syncthreads();
+                if (inb) {
+                    F[p] = nk;
+                    FV[p] = nv;
+                }
+                __syncthreads();
+            }
+            // the marks are counted by every later round, so they clear only now (the
+            // leader sets new ones after the next hop's first barriers)
+            for (int j = w; j < nChunks; j += WPQ) {
+                const int p = top - 64 * j + lane;
+                if (p < newSize) occ[p] = 0;
+            }
+            fsize = newSize;
+        }
+
+        // ---- all: cut, then the next node (reads only) ----
+        if (placed > 0 && c.k > 0 && fsize > c.k) {
+            const int dk = (int)(F[c.k] >> 32);
+            const float t = (float)(c.cut * (double)dk);
+            const int stride = (fsize + 63) >> 6;
+            auto pred = [&](int p) {
+                const uint64_t kv = F[p];
+                const float df = (float)(int)(kv >> 32);
+                return (t < df) || (t == df && (uint32_t)kv > 0u);
+            };
+            const int lo0 = lane * stride;
+            const uint64_t m = ballot(lo0 < fsize && pred(min(lo0 + stride, fsize) - 1));
+            if (m) {
+                const int f = __ffsll((unsigned long long)m) - 1;
+                const int lo = f * stride, hi = min(lo + stride, fsize);
+                int nf = hi - 1;
+                for (int p0 = lo; p0 < hi; p0 += 64) {
+                    const int p = p0 + lane;
+                    const uint64_t m2 = ballot(p < hi && pred(p));
+                    if (m2) {
+                        nf = p0 + __ffsll((unsigned long long)m2) - 1;
+                        break;
+                    }
+                }
+                fsize = nf;
+            }
+        }
+        u = -1;
+        for (int p0 = placed > 0 ? min(pmin, uPrev + 1) : uPrev + 1; p0 < fsize; p0 += 64) {
+            const int p = p0 + lane;
+            const bool nv = (p < fsize) && !FV[p];
+            const uint64_t m = ballot(nv);
+            if (m) {
+                u = p0 + __ffsll((unsigned long long)m) - 1;
+                break;
+            }
+        }
+        // every wavefront has read F / FV before the leader's next hop writes them
+        __syncthreads();
+    }
+
+    // ---- output by the leader: first k of the frontier ----
+    if (leader) {
+        if (fsize < c.k) flags |= kFlagShort;
+        uint32_t* oi = L.outIds + (size_t)q * L.outStride;
+        int32_t* od = L.outDists + (size_t)q * L.outStride;
+        for (int j = lane; j < (int)L.outStride; j += 64) {
+            if (j < c.k && j < fsize) {
+                const uint64_t kv = F[j];
+                oi[j] = (uint32_t)kv;
+                od[j] = (int32_t)(kv >> 32);
+            } else {
+                oi[j] = kEmpty;
+                od[j] = 0x7FFFFFFF;
+            }
+        }
+        if (lane == 0) {
+            if (L.outComps) L.outComps[q] = comps;
+            if (L.outFlags) L.outFlags[q] = flags;
+            if (L.outEdges) L.outEdges[q] = edges;
+            if (L.outHops) L.outHops[q] = hops;
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------
 // Register-resident form for Ls <= 64 (128 with two keys per lane) and rows of at most
 // 64 entries.  Same semantics as searchOne; lane i holds frontier entry i as one tagged
 // 64-bit key
@@ -1279,6 +1627,84 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     drainQueries<1, FAST>(L);
 }
 
+// The cooperative general form: blocks of WPQ wavefronts, one query per block at a time
+// (the queue ticket goes to the block through LDS), HBM scratch per block.
+template <int MAXCH, int WPQ, bool VG>
+__global__ __launch_bounds__(64 * WPQ) void beamFlatCoopKernel(FlatLaunch L) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t ticket;
+    const int lane = laneId();
+    const int w = threadIdx.x >> 6;
+    const uint32_t slot = blockIdx.x;
+    uint32_t* gfilt = L.gfilt ? L.gfilt + ((size_t)slot << L.gbitsMax) : nullptr;
+    uint32_t* gvk = VG ? L.gvkey + (size_t)slot * L.capV : nullptr;
+    unsigned char* gvc = VG ? L.gvcnt + (size_t)slot * L.capV : nullptr;
+    const uint32_t qEnd = L.queue ? kQueueEmpty : L.nq;
+    auto next = [&](uint32_t prev, bool first) -> uint32_t {
+        if (!L.queue) return first ? slot : prev + L.totalWaves;
+        if (w == 0) {
+            const uint32_t t = takeQuery(L.queue, L.order, L.nq, lane);
+            if (lane == 0) ticket = t;
+        }
+        __syncthreads();
+        const uint32_t t = ticket;
+        __syncthreads();
+        return t;
+    };
+    uint32_t q = next(0, true);
+    while (q < qEnd) {
+        QueryCfg c;
+        if (L.qp) {
+            const QueryParams& P = L.qp[q];
+            c.qid = P.qid;
+            c.Ls = P.Ls;
+            c.k = P.k;
+            c.limit = P.visit_limit > 0 ? P.visit_limit : (int64_t)L.n;
+            c.cut = P.cut;
+        } else {
+            c.qid = L.qids ? L.qids[q] : q;
+            c.Ls = L.Ls;
+            c.k = L.k;
+            c.limit = L.limit;
+            c.cut = L.cut;
+        }
+        makeUniform(c);
+        const int fb = flatFilterBits(c.Ls);
+        if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride ||
+            (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
+            if (w == 0) rejectQuery(L, q, lane);
+        } else if (fb <= L.fbitsLds) {
+            searchCoop<MAXCH, false, VG, WPQ>(L, smem, q, c, nullptr, gvk, gvc, w);
+        } else {
+            searchCoop<MAXCH, true, VG, WPQ>(L, smem, q, c, gfilt, gvk, gvc, w);
+        }
+        q = next(q, false);
+    }
+    if (L.queue && w == 0) finishQueue(L.queue, L.totalWaves, lane);
+}
+
+template <int WPQ, bool VG>
+hipError_t launchCoop(const FlatLaunch& L, int maxch, int blocks, hipStream_t stream) {
+    const dim3 grid(blocks), block(64 * WPQ);
+    switch (maxch) {
+        case 1: hipLaunchKernelGGL((beamFlatCoopKernel<1, WPQ, VG>), grid, block, L.ldsPerWave, stream, L); break;
+        case 2: hipLaunchKernelGGL((beamFlatCoopKernel<2, WPQ, VG>), grid, block, L.ldsPerWave, stream, L); break;
+        case 4: hipLaunchKernelGGL((beamFlatCoopKernel<4, WPQ, VG>), grid, block, L.ldsPerWave, stream, L); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int WPQ, bool VG>
+hipError_t occCoop(int maxch, size_t lds, int* bpc) {
+    switch (maxch) {
+        case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, beamFlatCoopKernel<1, WPQ, VG>, 64 * WPQ, lds);
+        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, beamFlatCoopKernel<2, WPQ, VG>, 64 * WPQ, lds);
+        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, beamFlatCoopKernel<4, WPQ, VG>, 64 * WPQ, lds);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace
 
 #ifdef PANN_PROFILE
@@ -1302,6 +1728,17 @@ hipError_t profileCounters(uint64_t*, int, bool) { return hipErrorNotSupported; 
 
 hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int blocks,
                           hipStream_t stream) {
+    if (L.wpq > 1) {
+        if (L.fast > 0 || wavesPerBlock != L.wpq) return hipErrorInvalidValue;
+        if (L.fast < 0 && (L.gvkey == nullptr || L.gvcnt == nullptr)) return hipErrorInvalidValue;
+        switch (L.wpq * 2 + (L.fast < 0 ? 1 : 0)) {
+            case 4: return launchCoop<2, false>(L, maxch, blocks, stream);
+            case 5: return launchCoop<2, true>(L, maxch, blocks, stream);
+            case 8: return launchCoop<4, false>(L, maxch, blocks, stream);
+            case 9: return launchCoop<4, true>(L, maxch, blocks, stream);
+            default: return hipErrorInvalidValue;
+        }
+    }
     const size_t lds = (size_t)L.ldsPerWave * wavesPerBlock;
     const dim3 grid(blocks), block(64 * wavesPerBlock);
     if (L.fast < 0) {
@@ -1349,6 +1786,16 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t beamFlatCoopOccupancy(int maxch, int wpq, bool hbmVisited, size_t ldsBytes, int* blocksPerCU) {
+    switch (wpq * 2 + (hbmVisited ? 1 : 0)) {
+        case 4: return occCoop<2, false>(maxch, ldsBytes, blocksPerCU);
+        case 5: return occCoop<2, true>(maxch, ldsBytes, blocksPerCU);
+        case 8: return occCoop<4, false>(maxch, ldsBytes, blocksPerCU);
+        case 9: return occCoop<4, true>(maxch, ldsBytes, blocksPerCU);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t beamFlatOccupancy(int maxch, int fast, int wavesPerBlock, size_t ldsBytes,

@@ -23,6 +23,7 @@ struct BatchPart {
     int fastKind = 0;  // flat: FlatLaunch::fast (0: general); HNSW: 1 with is_inw in HBM
     int wr = 1;        // HNSW: list width in wavefront registers
     int wavesPerBlock = 1, blocks = 0;
+    int wpq = 1;       // flat general form: wavefronts per query (blocks hold one query)
     uint32_t begin = 0, count = 0;
     size_t gfiltOff = 0, gvisOff = 0;  // this launch's HBM scratch within the batch's regions
     FlatLaunch L{};
@@ -78,6 +79,8 @@ int checkBatchConfig(const pann_batch_config* cfg) {
     if (cfg->visited_slots &&
         ((cfg->visited_slots & (cfg->visited_slots - 1)) || cfg->visited_slots < 64))
         return setError(PANN_ERR_ARG, "visited_slots must be 0 or a power of two >= 64");
+    if (cfg->waves_per_query > 4 || cfg->waves_per_query == 3)
+        return setError(PANN_ERR_ARG, "waves_per_query must be 0, 1, 2 or 4");
     if (cfg->prefetch > 1) return setError(PANN_ERR_ARG, "prefetch must be 0 or 1");
     return PANN_OK;
 }
@@ -159,13 +162,29 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int*
     P->LsLo = LsLo;
     P->LsHi = LsHi;
     P->fastKind = fast ? mode + (c.prefetch ? 2 : 0) : fast2 ? 8 + mode : hbmVisited ? -1 : 0;
-    const FlatLds lay = fast ? flatFastLdsLayout(capV)
-                      : fast2 ? flatFast2LdsLayout(capV)
-                              : flatLdsLayout(LsHi, b->maxch, hbmVisited ? 0 : capV, fbLaunch);
+    // several wavefronts per query: the general form only
+    const int wpq = (!fast && !fast2 && c.waves_per_query > 1) ? (int)c.waves_per_query : 1;
+    FlatLds lay = fast ? flatFastLdsLayout(capV)
+                : fast2 ? flatFast2LdsLayout(capV)
+                        : flatLdsLayout(LsHi, b->maxch, hbmVisited ? 0 : capV, fbLaunch);
+    if (wpq > 1) lay = flatCoopLayout(lay, b->maxch);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsHi, lay.total);
-    int r = sizeLaunch(b, P->fastKind, 1, lay.total, &P->wavesPerBlock, &P->blocks);
-    if (r != PANN_OK) return r;
+    int r = PANN_OK;
+    if (wpq > 1) {
+        int cus = 0, bpc = 0;
+        r = deviceCUs(&cus);
+        if (r != PANN_OK) return r;
+        const hipError_t e = beamFlatCoopOccupancy(b->maxch, wpq, hbmVisited, lay.total, &bpc);
+        if (e != hipSuccess) return setError(PANN_ERR_HIP, "occupancy query: %s", hipGetErrorString(e));
+        bpc = std::min(bpc, ldsBlocksPerCU(lay.total));
+        P->wavesPerBlock = wpq;
+        P->blocks = std::max(1, bpc) * cus;
+    } else {
+        r = sizeLaunch(b, P->fastKind, 1, lay.total, &P->wavesPerBlock, &P->blocks);
+        if (r != PANN_OK) return r;
+    }
+    P->wpq = wpq;
     *gbits = (fbMax > fbLds || fast2) ? fbMax : 0;
     FlatLaunch& L = P->L;
     L.n = b->graph->n;
@@ -177,6 +196,7 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int*
     L.ldsPerWave = lay.total;
     L.lds = lay;
     L.fast = P->fastKind;
+    L.wpq = wpq;
     if (top) {
         b->cfg.lds_filter_bits = fbLds;
         b->cfg.visited_slots = capV;
@@ -430,7 +450,7 @@ int pann_batch_set_queries(pann_batch* b, const int8_t* host_rows, const pann_qu
     // every query then starts at once, and the batch takes as long as its slowest query
     // rather than the sum of each class's slowest.
     BatchPart& top = b->parts.back();
-    if (nq <= (uint32_t)top.blocks * top.wavesPerBlock) {
+    if (nq <= (uint32_t)top.blocks * (top.wpq > 1 ? 1u : (uint32_t)top.wavesPerBlock)) {
         for (BatchPart& P : b->parts) P.begin = P.count = 0;
         top.count = nq;
     } else {
@@ -466,9 +486,9 @@ int pann_batch_launch(pann_batch* b, void* stream) {
         const bool fork = used > 1 && i + 1 < b->parts.size();
         const hipStream_t ps = fork ? b->side[i] : st;
         if (fork) HIP_TRY(hipStreamWaitEvent(ps, b->forkEv, 0));
-        const int blocks = (int)std::min<uint32_t>((uint32_t)P.blocks,
-                                                   (P.count + P.wavesPerBlock - 1) / P.wavesPerBlock);
-        const uint32_t waves = (uint32_t)blocks * P.wavesPerBlock;
+        const uint32_t qpb = P.wpq > 1 ? 1u : (uint32_t)P.wavesPerBlock;  // queries per block
+        const int blocks = (int)std::min<uint32_t>((uint32_t)P.blocks, (P.count + qpb - 1) / qpb);
+        const uint32_t waves = (uint32_t)blocks * qpb;  // query slots: the queue's finishers
         if (b->kind == 0) {
             FlatLaunch L = P.L;
             L.nq = P.count;
@@ -587,7 +607,13 @@ int pann_batch_autotune(const pann_vectors* base, const pann_graph* g, const pan
                 if (!h && vs < 2u * (uint32_t)max_Ls + 64) continue;
                 for (uint32_t pf : {0u, 1u}) {
                     if (pf && (h || max_Ls > kFastLsMax)) continue;
-                    cands.push_back({max_Ls, max_k, wpb, fb, vs, pf});
+                    // several wavefronts per query: flat graphs whose top class is the
+                    // general form (beams above 128 or rows above 64)
+                    const bool general = !h && (max_Ls > kFast2LsMax || g->G > 64);
+                    for (uint32_t wq : {1u, 2u, 4u}) {
+                        if (wq > 1 && (!general || wpb != 1)) continue;
+                        cands.push_back({max_Ls, max_k, wpb, fb, vs, pf, wq});
+                    }
                 }
             }
         }

@@ -43,7 +43,9 @@ struct FlatLds {
     uint32_t vkey;     // visited hash keys (capV u32)
     uint32_t vcnt;     // visited hash counts (capV u8)
     uint32_t filt;     // seen filter (1 << fbitsLds u32), when held in LDS
-    uint32_t total;    // bytes per wave
+    uint32_t ckey;     // cooperative form: candidate keys of the hop (64*maxch u64)
+    uint32_t cscal;    // cooperative form: the leader's published scalars (8 x i32)
+    uint32_t total;    // bytes per wave (per query in the cooperative form)
 };
 
 __host__ __device__ inline uint32_t alignUp(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
@@ -63,6 +65,18 @@ __host__ __device__ inline FlatLds flatLdsLayout(int LsMax, int maxch, uint32_t 
     L.kvis = o;    o += alignUp(cmax, 8);
     L.tag = o;     o += kTagSlots;
     L.vcnt = o;    o += alignUp(capV, 8);
+    L.ckey = 0;
+    L.cscal = 0;
+    L.total = alignUp(o, 16);
+    return L;
+}
+
+// The cooperative general form (2 or 4 wavefronts per query) appends the candidate-key
+// buffer its wavefronts fill and the scalars its leader publishes.
+__host__ __device__ inline FlatLds flatCoopLayout(FlatLds L, int maxch) {
+    uint32_t o = L.total;
+    L.ckey = o;  o += 8u * 64u * (uint32_t)maxch;
+    L.cscal = o; o += 32u;
     L.total = alignUp(o, 16);
     return L;
 }
@@ -154,6 +168,9 @@ struct FlatLaunch {
                                // form (flatFast2LdsLayout, filter in gfilt): 9, or 10 below
                                // 4 GiB (the launcher adds 2 for normsInRow).  0: general;
                                // -1: general with the visited table in HBM (gvkey/gvcnt)
+    int32_t wpq;               // general form: wavefronts per query (0 or 1: one; 2 or 4:
+                               // the cooperative kernel, one query per block, scratch per
+                               // block, layout flatCoopLayout)
     uint32_t* gfilt;           // per-wave global filter regions
     uint32_t* gvkey;           // fast -1: capV visited keys per wave (then the counts:
     unsigned char* gvcnt;      //          capV bytes per wave)
@@ -256,6 +273,9 @@ hipError_t beamHnswOccupancy(int wr, int maxch, bool inwHbm, int wavesPerBlock, 
 // Launchers implemented in the .hip translation units.
 hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int blocks,
                           hipStream_t stream);
+// Blocks of the cooperative general form (wpq wavefronts, one query, ldsBytes of LDS)
+// resident per CU.
+hipError_t beamFlatCoopOccupancy(int maxch, int wpq, bool hbmVisited, size_t ldsBytes, int* blocksPerCU);
 hipError_t beamFlatOccupancy(int maxch, int fast, int wavesPerBlock, size_t ldsBytes,
                              int* blocksPerCU);
 // Squared norms of n rows into out; with embed, over bytes 0..123 only, also written

@@ -48,7 +48,7 @@ class SearchOutputs(C.Structure):
 class BatchConfig(C.Structure):
     _fields_ = [("max_Ls", C.c_int32), ("max_k", C.c_int32), ("waves_per_block", C.c_int32),
                 ("lds_filter_bits", C.c_int32), ("visited_slots", C.c_uint32),
-                ("prefetch", C.c_uint32)]
+                ("prefetch", C.c_uint32), ("waves_per_query", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):

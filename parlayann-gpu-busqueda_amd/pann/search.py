@@ -305,9 +305,10 @@ class Batch:
     launch of resident wavefronts; optionally captured into a HIP graph and replayed."""
 
     def __init__(self, max_queries, max_Ls, max_k, base=None, graph=None, hnsw=None,
-                 waves_per_block=0, lds_filter_bits=0, visited_slots=0, prefetch=0, config=None):
+                 waves_per_block=0, lds_filter_bits=0, visited_slots=0, prefetch=0, config=None,
+                 waves_per_query=0):
         cfg = config or _lib.BatchConfig(max_Ls, max_k, waves_per_block, lds_filter_bits,
-                                          visited_slots, prefetch)
+                                          visited_slots, prefetch, waves_per_query)
         h = C.c_void_p()
         if hnsw is not None:
             check(lib().pann_batch_prepare_hnsw(hnsw.handle, max_queries, C.byref(cfg), C.byref(h)))

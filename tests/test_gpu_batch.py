@@ -244,3 +244,59 @@ def test_ls_class_launches(gpu, w1):
     o = O.beam_flat(b2, g2, q2, _oracle_params(params))
     _check_against_oracle(r, o, params, "G=100")
     batch.free(), g.free(), b.free()
+
+
+@pytest.mark.parametrize("graph_kind", ["vamana", "wide_rows"])
+def test_waves_per_query_identical(gpu, small, graph_kind):
+    """1, 2 and 4 wavefronts per query give the same outputs (ids, distances, counts,
+    flags, hops) as each other and as the oracle: beams 129..400 on the Vamana graph (the
+    general form with its visited table in HBM), and Ls 10..300 on rows of up to 100
+    entries with duplicates (the general form for every Ls)."""
+    from conftest import random_graph
+    from oracle import oracle as O
+    from pann import search
+    base, queries, graph = small
+    rng = np.random.default_rng(5)
+    if graph_kind == "wide_rows":
+        graph = random_graph(len(base), 100, rng, dup_frac=0.1)
+        params = _mixed_params(len(queries), len(base), rng, ls_lo=10, ls_hi=300)
+        max_Ls = 300
+    else:
+        params = _mixed_params(len(queries), len(base), rng, ls_lo=129, ls_hi=400)
+        max_Ls = 400
+    b, g = search.Vectors(base), search.FlatGraph(graph)
+    o = O.beam_flat(base, graph, queries, _oracle_params(params))
+    outs = {}
+    for wq in (1, 2, 4):
+        print(f"[waves/query] {graph_kind} wq={wq}", flush=True)
+        batch = search.Batch(len(queries), max_Ls, 100, base=b, graph=g, waves_per_query=wq)
+        assert batch.config()[0]["waves_per_query"] == wq
+        batch.set_queries(queries, params)
+        batch.launch()
+        search.synchronize()
+        outs[wq] = batch.download()
+        batch.free()
+        _check_against_oracle(outs[wq], o, params, f"{graph_kind} waves/query {wq}")
+    for wq in (2, 4):
+        for f in ("ids", "dists", "comps", "flags", "edges", "hops"):
+            assert np.array_equal(getattr(outs[wq], f), getattr(outs[1], f)), (graph_kind, wq, f)
+    g.free(), b.free()
+
+
+def test_autotune_sweeps_waves_per_query(gpu, small):
+    """The autotuner's candidates include 2 and 4 wavefronts per query for long beams; the
+    configuration it returns runs cleanly and names its waves per query."""
+    from pann import search
+    base, queries, graph = small
+    rng = np.random.default_rng(6)
+    params = _mixed_params(len(queries), len(base), rng, ls_lo=150, ls_hi=300)
+    b, g = search.Vectors(base), search.FlatGraph(graph)
+    cfg, ms = search.autotune(queries, params, 300, 100, base=b, graph=g)
+    assert cfg.waves_per_query in (1, 2, 4) and ms > 0
+    batch = search.Batch(len(queries), 300, 100, base=b, graph=g, config=cfg)
+    batch.set_queries(queries, params)
+    batch.launch()
+    search.synchronize()
+    r = batch.download()
+    assert not (r.flags & (search.FLAG_OVERFLOW | search.FLAG_CONFIG)).any()
+    batch.free(), g.free(), b.free()
