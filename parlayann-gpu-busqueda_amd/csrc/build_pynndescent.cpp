@@ -6,7 +6,9 @@
 // (exact K nearest inside each leaf of at most leaf_size points), refined by NN-descent
 // local joins (new x new and new x old neighbour pairs, reverse lists included) until an
 // iteration changes fewer than delta * n * K entries, then every node's K-NN list plus its
-// reverse edges is robust-pruned with alpha down to max_degree (as Vamana prunes).
+// reverse edges is robust-pruned with alpha down to max_degree (as Vamana prunes).  The
+// pruned graph's components are then joined inside the cluster-tree leaves by their
+// closest pairs (step 4), which keeps the graph searchable from node 0 on clustered data.
 //
 // Each K-NN list keeps the K best (distance, id) pairs of everything proposed to it, so
 // the lists after an iteration do not depend on the order in which threads propose; the
@@ -175,7 +177,88 @@ extern "C" int pann_build_pynndescent(const int8_t* base, uint32_t n, uint32_t d
         for (uint32_t r : rev[i]) cand.push_back({distI8(vec((uint32_t)i), vec(r), dim), r});
         robustPrune(base, dim, (uint32_t)i, cand, alpha, max_degree, adj[i]);
     }, 64);
-    // a K-NN graph of well-separated clusters is disconnected: bridge it from node 0
+    // 4. leaf bridges.  A K-NN graph of well-separated clusters falls apart into one
+    // component per cluster, and a search that starts at node 0 never leaves its own.
+    // The cluster trees of step 1 put pieces of neighbouring clusters into one leaf; in
+    // every such leaf the pruned graph's components are joined the way a minimum spanning
+    // tree joins them: the closest pair between each two components present, taken by
+    // Kruskal in (distance, endpoints) order over the leaf's components, each accepted
+    // pair added in both directions.  Each leaf's Kruskal is its own, so a cluster keeps
+    // the bridges of every tree, as HCNNG's per-leaf spanning trees do.
+    std::vector<uint32_t> comp(n);
+    {
+        std::vector<uint32_t> p(n);
+        std::iota(p.begin(), p.end(), 0u);
+        auto find = [&](uint32_t x) {
+            while (p[x] != x) x = p[x] = p[p[x]];
+            return x;
+        };
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t a : adj[i]) {
+                const uint32_t ra = find(i), rb = find(a);
+                if (ra != rb) p[std::max(ra, rb)] = std::min(ra, rb);
+            }
+        for (uint32_t i = 0; i < n; i++) comp[i] = find(i);
+    }
+    for (uint32_t t = 0; t < num_trees; t++) {
+        std::vector<uint32_t> ids(n);
+        std::iota(ids.begin(), ids.end(), 0u);
+        const auto leaves = clusterTreeLeaves(base, dim, ids, leaf_size, mixSeed(seed, 200 + t), nthreads);
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> found(leaves.size());
+        parallelFor(leaves.size(), nthreads, [&](size_t li, int) {
+            const uint32_t b = leaves[li].first, m = leaves[li].second - b;
+            const uint32_t* L = ids.data() + b;
+            std::vector<uint32_t> cs(m);
+            for (uint32_t i = 0; i < m; i++) cs[i] = comp[L[i]];
+            std::vector<uint32_t> distinct(cs);
+            std::sort(distinct.begin(), distinct.end());
+            distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
+            const uint32_t C = (uint32_t)distinct.size();
+            if (C < 2) return;
+            for (auto& c : cs) c = (uint32_t)(std::lower_bound(distinct.begin(), distinct.end(), c) - distinct.begin());
+            // closest pair per component pair, (distance, smaller id, larger id) order
+            struct Pair {
+                int32_t d;
+                uint32_t x, y;
+                bool operator<(const Pair& o) const {
+                    return d != o.d ? d < o.d : x != o.x ? x < o.x : y < o.y;
+                }
+            };
+            std::vector<Pair> best((size_t)C * C, Pair{INT32_MAX, 0xFFFFFFFFu, 0xFFFFFFFFu});
+            for (uint32_t i = 0; i < m; i++)
+                for (uint32_t j = i + 1; j < m; j++) {
+                    if (cs[i] == cs[j]) continue;
+                    const uint32_t x = std::min(L[i], L[j]), y = std::max(L[i], L[j]);
+                    const Pair c{distI8(vec(x), vec(y), dim), x, y};
+                    Pair& slot = best[(size_t)std::min(cs[i], cs[j]) * C + std::max(cs[i], cs[j])];
+                    if (c < slot) slot = c;
+                }
+            std::vector<std::pair<Pair, std::pair<uint32_t, uint32_t>>> edges;
+            for (uint32_t a = 0; a < C; a++)
+                for (uint32_t c = a + 1; c < C; c++)
+                    if (best[(size_t)a * C + c].d != INT32_MAX) edges.push_back({best[(size_t)a * C + c], {a, c}});
+            std::sort(edges.begin(), edges.end(),
+                      [](const auto& u, const auto& v) { return u.first < v.first; });
+            std::vector<uint32_t> p(C);
+            std::iota(p.begin(), p.end(), 0u);
+            auto find = [&](uint32_t x) {
+                while (p[x] != x) x = p[x] = p[p[x]];
+                return x;
+            };
+            for (const auto& e : edges) {
+                const uint32_t ra = find(e.second.first), rb = find(e.second.second);
+                if (ra == rb) continue;
+                p[ra] = rb;
+                found[li].push_back({e.first.x, e.first.y});
+            }
+        }, 1);
+        for (const auto& f : found)
+            for (const auto& e : f) {
+                adj[e.first].push_back(e.second);
+                adj[e.second].push_back(e.first);
+            }
+    }
+    // whatever the leaves left unreachable from node 0
     bridgeFromStart(base, n, dim, adj, nthreads);
     return writeGraphLists(out_path, adj);
 }

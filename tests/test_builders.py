@@ -81,3 +81,22 @@ def test_pynndescent_structure(data):
                                     nthreads=1)
     assert np.array_equal(g.degrees, g1.degrees) and np.array_equal(g.edges, g1.edges)
     assert _reachable(g)
+
+
+def test_pynndescent_clustered_data_is_searchable():
+    """On a mixture whose clusters are far apart every K-NN list stays inside its cluster;
+    the leaf bridges must make the graph strongly connected, so a search from node 0
+    reaches every cluster (recall measured with the oracle against exact neighbours)."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    from oracle import oracle as O
+    from pann import builders, datagen
+    base, queries = datagen.gmm_dataset(6000, 200, dim=100, ncomp=24, seeds=(31, 32, 33))
+    g = builders.build_pynndescent(base, K=16, num_trees=4, leaf_size=400, seed=2, nthreads=4)
+    rows = np.repeat(np.arange(g.n), g.degrees.astype(np.int64))
+    m = csr_matrix((np.ones(len(g.edges)), (rows, g.edges.astype(np.int64))), shape=(g.n, g.n))
+    assert connected_components(m, directed=True, connection="strong")[0] == 1
+    gt_ids, gt_d = O.bruteforce_knn(base, queries, 10, nthreads=4)
+    o = O.beam_flat(base, g, queries, O.make_params(len(queries), 50, 10), nthreads=4)
+    assert O.recall(gt_ids, gt_d, o.ids, 10) >= 0.9
