@@ -341,6 +341,19 @@ int pann_build_vamana_gpu(const pann_vectors* base, uint32_t R, uint32_t L, floa
 int pann_build_hnsw_gpu(const pann_vectors* base, uint32_t M, uint32_t efc, float ml, float alpha,
                         uint64_t seed, const char* out_path, void* stream);
 
+/* HCNNG graph over the rows of `base`, built on the device and written as a ParlayANN
+ * graph file: num_trees random cluster trees (two pivots per split, leaves of at most
+ * leaf_size points), in every leaf each point's 16 nearest leaf mates and a spanning
+ * forest of node degree at most mst_deg over those edges (Kruskal), a node's list being
+ * the union over trees of its forest edges; nodes node 0 cannot reach get one edge from
+ * their nearest reachable node.  This is the reference's GPU plan for HCNNG
+ * (README.md:351-353) and otherwise loads the prebuilt graph (leerGrafo,
+ * cuda/lectores.cpp:97-144).  The file equals the one pann_build_hcnng
+ * (include/pann_tools.h) writes for the same arguments, byte for byte.  leaf_size <= 1024,
+ * mst_deg <= 4.  Blocking. */
+int pann_build_hcnng_gpu(const pann_vectors* base, uint32_t num_trees, uint32_t leaf_size,
+                         uint32_t mst_deg, uint64_t seed, const char* out_path, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-std=c++17", "-O3", "-fPIC", "-march=x86-64-v3", "-pthread", "-Wall",
               f"-I{os.path.join(REPO, 'include')}"]
 
-PANN_HIP_SOURCES = ["beam_flat.hip", "beam_hnsw.hip", "bf_knn.hip", "rows.hip", "build_gpu.hip", "pann_api.cpp", "pann_batch.cpp", "pann_server.cpp", "pann_noalloc.cpp", "formats.cpp"]
+PANN_HIP_SOURCES = ["beam_flat.hip", "beam_hnsw.hip", "bf_knn.hip", "rows.hip", "build_gpu.hip", "build_gpu_trees.hip", "pann_api.cpp", "pann_batch.cpp", "pann_server.cpp", "pann_noalloc.cpp", "formats.cpp"]
 TOOLS_SOURCES = ["tools.cpp", "build_vamana.cpp", "build_hnsw.cpp", "build_hcnng.cpp",
                  "build_pynndescent.cpp", "formats.cpp"]
 

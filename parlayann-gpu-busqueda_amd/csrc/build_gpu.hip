@@ -30,6 +30,7 @@
 // Prunes run in two tiers: four wavefronts per block with room for 512 candidates
 // each, and the rare wider prunes (up to 2048 candidates) deferred to a second launch
 // with one wavefront per block.
+#include "devmem.h"
 #include "device_common.h"
 #include "hnsw_plan.h"
 #include "tools_common.h"
@@ -780,26 +781,6 @@ __global__ __launch_bounds__(64 * WPB) void hnswBuildReverseKernel(HnswBuildArgs
     }
 }
 
-struct DevMem {
-    std::vector<void*> ptrs;
-    ~DevMem() {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-    template <class T>
-    hipError_t alloc(T** p, size_t count) {
-        void* q = nullptr;
-        const hipError_t e = hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T));
-        if (e == hipSuccess) ptrs.push_back(q);
-        *p = static_cast<T*>(q);
-        return e;
-    }
-};
-
-#define BUILD_TRY(x)                        \
-    do {                                    \
-        const hipError_t e_ = (x);          \
-        if (e_ != hipSuccess) return e_;    \
-    } while (0)
 
 hipError_t launchSearch(const BuildArgs& a, int C, uint32_t blocks, hipStream_t s) {
     switch (C) {

@@ -184,8 +184,12 @@ extern "C" int pann_build_pynndescent(const int8_t* base, uint32_t n, uint32_t d
     // tree joins them: the closest pair between each two components present, taken by
     // Kruskal in (distance, endpoints) order over the leaf's components, each accepted
     // pair added in both directions.  Each leaf's Kruskal is its own, so a cluster keeps
-    // the bridges of every tree, as HCNNG's per-leaf spanning trees do.
-    std::vector<uint32_t> comp(n);
+    // the bridges of every tree, as HCNNG's per-leaf spanning trees do.  A node takes part
+    // in at most kBridgeCap bridges (pairs are drawn from the nodes with room left when
+    // the tree starts, and a pair whose endpoint filled up in this leaf is skipped), so
+    // rows stay within max_degree + kBridgeCap entries.
+    constexpr uint32_t kBridgeCap = 24;
+    std::vector<uint32_t> comp(n), bridges(n, 0);
     {
         std::vector<uint32_t> p(n);
         std::iota(p.begin(), p.end(), 0u);
@@ -227,7 +231,7 @@ extern "C" int pann_build_pynndescent(const int8_t* base, uint32_t n, uint32_t d
             std::vector<Pair> best((size_t)C * C, Pair{INT32_MAX, 0xFFFFFFFFu, 0xFFFFFFFFu});
             for (uint32_t i = 0; i < m; i++)
                 for (uint32_t j = i + 1; j < m; j++) {
-                    if (cs[i] == cs[j]) continue;
+                    if (cs[i] == cs[j] || bridges[L[i]] >= kBridgeCap || bridges[L[j]] >= kBridgeCap) continue;
                     const uint32_t x = std::min(L[i], L[j]), y = std::max(L[i], L[j]);
                     const Pair c{distI8(vec(x), vec(y), dim), x, y};
                     Pair& slot = best[(size_t)std::min(cs[i], cs[j]) * C + std::max(cs[i], cs[j])];
@@ -245,10 +249,19 @@ extern "C" int pann_build_pynndescent(const int8_t* base, uint32_t n, uint32_t d
                 while (p[x] != x) x = p[x] = p[p[x]];
                 return x;
             };
+            std::vector<uint32_t> used(m, 0);  // bridges taken in this leaf, by leaf index
+            auto room = [&](uint32_t x) {
+                const uint32_t k = (uint32_t)(std::find(L, L + m, x) - L);
+                return bridges[x] + used[k] < kBridgeCap ? k : m;
+            };
             for (const auto& e : edges) {
                 const uint32_t ra = find(e.second.first), rb = find(e.second.second);
                 if (ra == rb) continue;
+                const uint32_t kx = room(e.first.x), ky = room(e.first.y);
+                if (kx == m || ky == m) continue;
                 p[ra] = rb;
+                used[kx]++;
+                used[ky]++;
                 found[li].push_back({e.first.x, e.first.y});
             }
         }, 1);
@@ -256,6 +269,8 @@ extern "C" int pann_build_pynndescent(const int8_t* base, uint32_t n, uint32_t d
             for (const auto& e : f) {
                 adj[e.first].push_back(e.second);
                 adj[e.second].push_back(e.first);
+                bridges[e.first]++;
+                bridges[e.second]++;
             }
     }
     // whatever the leaves left unreachable from node 0

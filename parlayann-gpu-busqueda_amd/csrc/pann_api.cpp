@@ -12,6 +12,9 @@
 #include "formats.h"
 #include "hnsw_plan.h"
 #include "pann_objects.h"
+#include "tools_common.h"
+
+#include <thread>
 
 using namespace pann;
 
@@ -788,6 +791,25 @@ int pann_bruteforce_knn(const pann_vectors* base, const pann_vectors* queries, u
     return PANN_OK;
 }
 
+// Joins what node 0 cannot reach (as the host builders do, tools::bridgeFromStart: the
+// base rows are copied back only when a bridge is needed) and writes the graph file.
+static int bridgeAndWrite(const pann_vectors* base, std::vector<std::vector<uint32_t>>& adj,
+                          const char* out_path) {
+    if (!tools::allReachableFromStart(adj)) {
+        const uint32_t n = base->n, dim = base->dim;
+        std::vector<int8_t> rows((size_t)n * kVecBytes), compact((size_t)n * dim);
+        const hipError_t e = hipMemcpy(rows.data(), base->dev, rows.size(), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return setError(PANN_ERR_HIP, "copy of the base rows: %s", hipGetErrorString(e));
+        for (uint32_t i = 0; i < n; i++)
+            std::copy_n(rows.data() + (size_t)i * kVecBytes, dim, compact.data() + (size_t)i * dim);
+        tools::bridgeFromStart(compact.data(), n, dim, adj, (int)std::max(1u, std::thread::hardware_concurrency()));
+    }
+    const int w = tools::writeGraphListsFile(out_path, adj);
+    if (w == 1) return setError(PANN_ERR_IO, "cannot write %s", out_path);
+    if (w != 0) return setError(PANN_ERR_IO, "short write to %s", out_path);
+    return PANN_OK;
+}
+
 int pann_build_vamana_gpu(const pann_vectors* base, uint32_t R, uint32_t L, float alpha,
                           int two_pass, uint64_t seed, uint32_t* out_degrees,
                           uint32_t* out_edges, void* stream) {
@@ -807,6 +829,22 @@ int pann_build_vamana_gpu(const pann_vectors* base, uint32_t R, uint32_t L, floa
     if (status & 1u) return setError(PANN_ERR_CAPACITY, "a build search expanded more than 1024 nodes");
     if (status & 2u) return setError(PANN_ERR_CAPACITY, "a robust prune met more than 2048 candidates");
     return PANN_OK;
+}
+
+int pann_build_hcnng_gpu(const pann_vectors* base, uint32_t num_trees, uint32_t leaf_size,
+                         uint32_t mst_deg, uint64_t seed, const char* out_path, void* stream) {
+    if (!base || !out_path) return setError(PANN_ERR_ARG, "NULL argument");
+    if (base->n == 0) return setError(PANN_ERR_ARG, "no points");
+    if (num_trees == 0) return setError(PANN_ERR_ARG, "num_trees must be positive");
+    if (leaf_size < 2 || leaf_size > 1024)
+        return setError(PANN_ERR_CAPACITY, "leaf_size=%u outside 2..1024", leaf_size);
+    if (mst_deg == 0 || mst_deg > 4) return setError(PANN_ERR_CAPACITY, "mst_deg=%u outside 1..4", mst_deg);
+    std::vector<std::vector<uint32_t>> adj;
+    const hipError_t e = launchBuildHcnng(base->dev, base->dim <= (uint32_t)kNormInRowMaxDim, base->n,
+                                          num_trees, leaf_size, mst_deg, seed, adj, (hipStream_t)stream);
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "HCNNG build: %s", hipGetErrorString(e));
+    const int r = bridgeAndWrite(base, adj, out_path);
+    return r;
 }
 
 int pann_build_hnsw_gpu(const pann_vectors* base, uint32_t M, uint32_t efc, float ml, float alpha,

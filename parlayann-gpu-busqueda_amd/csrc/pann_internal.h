@@ -308,6 +308,10 @@ hipError_t launchBuildHnsw(const int8_t* vec, bool normsInRow, uint32_t n, uint3
 hipError_t launchBuildVamana(const int8_t* vec, bool normsInRow, uint32_t n, uint32_t R, uint32_t L,
                              float alpha, bool twoPass, uint64_t seed, int cus, uint32_t* outDeg,
                              uint32_t* outAdj, uint32_t* status, hipStream_t s);
+// HCNNG over the n device rows of vec: adj[i] = node i's list, before any bridging.
+hipError_t launchBuildHcnng(const int8_t* vec, bool normsInRow, uint32_t n, uint32_t numTrees,
+                            uint32_t leafSize, uint32_t mstDeg, uint64_t seed,
+                            std::vector<std::vector<uint32_t>>& adj, hipStream_t s);
 hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_t n,
                                const int8_t* queries, const int32_t* qnorm, uint32_t nq,
                                uint32_t k, bool normsInRow, uint32_t* outIds, int32_t* outDists,

@@ -32,59 +32,10 @@ int writeParlayGraph(const char* path, uint32_t n, uint32_t maxDeg, const uint32
     return 0;
 }
 
-size_t bridgeFromStart(const int8_t* base, uint32_t n, uint32_t dim,
-                       std::vector<std::vector<uint32_t>>& adj, int nthreads) {
-    std::vector<uint8_t> reach(n, 0);
-    std::vector<uint32_t> stack;
-    auto flood = [&](uint32_t s) {
-        reach[s] = 1;
-        stack.push_back(s);
-        while (!stack.empty()) {
-            const uint32_t u = stack.back();
-            stack.pop_back();
-            for (uint32_t a : adj[u])
-                if (!reach[a]) {
-                    reach[a] = 1;
-                    stack.push_back(a);
-                }
-        }
-    };
-    flood(0);
-    size_t added = 0;
-    const int T = std::max(1, nthreads);
-    for (uint32_t x = 0; x < n; x++) {
-        if (reach[x]) continue;
-        std::vector<Nbr> best(T, Nbr{INT32_MAX, 0xFFFFFFFFu});
-        const int8_t* xv = base + (size_t)x * dim;
-        parallelFor(n, T, [&](size_t y, int t) {
-            if (!reach[y]) return;
-            const Nbr c{distI8(xv, base + y * dim, dim), (uint32_t)y};
-            if (c < best[t]) best[t] = c;
-        }, 4096);
-        const Nbr b = *std::min_element(best.begin(), best.end());
-        adj[b.id].push_back(x);
-        added++;
-        flood(x);
-    }
-    return added;
-}
-
 int writeGraphLists(const char* path, const std::vector<std::vector<uint32_t>>& adj) {
-    const uint32_t n = (uint32_t)adj.size();
-    uint32_t maxDeg = 0;
-    for (const auto& a : adj) maxDeg = std::max<uint32_t>(maxDeg, (uint32_t)a.size());
-    FILE* f = fopen(path, "wb");
-    if (!f) return setError(2, std::string("cannot write ") + path);
-    const uint32_t hdr[2] = {n, maxDeg};
-    bool ok = fwrite(hdr, 4, 2, f) == 2;
-    for (uint32_t i = 0; ok && i < n; i++) {
-        const uint32_t d = (uint32_t)adj[i].size();
-        ok = fwrite(&d, 4, 1, f) == 1;
-    }
-    for (uint32_t i = 0; ok && i < n; i++)
-        ok = fwrite(adj[i].data(), 4, adj[i].size(), f) == adj[i].size();
-    ok = (fclose(f) == 0) && ok;
-    if (!ok) return setError(2, std::string("short write to ") + path);
+    const int r = writeGraphListsFile(path, adj);
+    if (r == 1) return setError(2, std::string("cannot write ") + path);
+    if (r == 2) return setError(2, std::string("short write to ") + path);
     return 0;
 }
 

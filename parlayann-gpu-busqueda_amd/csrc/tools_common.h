@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
+#include <cstdio>
 #include <string>
 #include <thread>
 #include <vector>
@@ -79,12 +81,6 @@ void parallelFor(size_t n, int nthreads, F&& body, size_t chunk = 16) {
 int writeParlayGraph(const char* path, uint32_t n, uint32_t maxDeg, const uint32_t* degrees,
                      const uint32_t* dense, uint32_t stride);
 
-// The search always starts at node 0.  For every node not reachable from 0 (in id order),
-// adds one edge from its nearest reachable node, then marks what became reachable.
-// Returns the number of edges added.
-size_t bridgeFromStart(const int8_t* base, uint32_t n, uint32_t dim,
-                       std::vector<std::vector<uint32_t>>& adj, int nthreads);
-
 int writeGraphLists(const char* path, const std::vector<std::vector<uint32_t>>& adj);
 
 struct Nbr {
@@ -93,6 +89,74 @@ struct Nbr {
     bool operator<(const Nbr& o) const { return d < o.d || (d == o.d && id < o.id); }
     bool operator==(const Nbr& o) const { return d == o.d && id == o.id; }
 };
+
+// Marks every node reachable from s (depth-first over the out-lists).
+inline void floodFrom(uint32_t s, const std::vector<std::vector<uint32_t>>& adj,
+                      std::vector<uint8_t>& reach) {
+    std::vector<uint32_t> stack{s};
+    reach[s] = 1;
+    while (!stack.empty()) {
+        const uint32_t u = stack.back();
+        stack.pop_back();
+        for (uint32_t a : adj[u])
+            if (!reach[a]) {
+                reach[a] = 1;
+                stack.push_back(a);
+            }
+    }
+}
+
+inline bool allReachableFromStart(const std::vector<std::vector<uint32_t>>& adj) {
+    std::vector<uint8_t> reach(adj.size(), 0);
+    if (!adj.empty()) floodFrom(0, adj, reach);
+    return std::find(reach.begin(), reach.end(), 0) == reach.end();
+}
+
+// The search always starts at node 0.  For every node not reachable from 0 (in id order),
+// adds one edge from its nearest reachable node (rows of `dim` bytes), then marks what
+// became reachable.  Returns the number of edges added.
+inline size_t bridgeFromStart(const int8_t* base, uint32_t n, uint32_t dim,
+                              std::vector<std::vector<uint32_t>>& adj, int nthreads) {
+    std::vector<uint8_t> reach(n, 0);
+    floodFrom(0, adj, reach);
+    size_t added = 0;
+    const int T = std::max(1, nthreads);
+    for (uint32_t x = 0; x < n; x++) {
+        if (reach[x]) continue;
+        std::vector<Nbr> best(T, Nbr{INT32_MAX, 0xFFFFFFFFu});
+        const int8_t* xv = base + (size_t)x * dim;
+        parallelFor(n, T, [&](size_t y, int t) {
+            if (!reach[y]) return;
+            const Nbr c{distI8(xv, base + y * dim, dim), (uint32_t)y};
+            if (c < best[t]) best[t] = c;
+        }, 4096);
+        const Nbr b = *std::min_element(best.begin(), best.end());
+        adj[b.id].push_back(x);
+        added++;
+        floodFrom(x, adj, reach);
+    }
+    return added;
+}
+
+// ParlayANN graph file from per-node lists ([n][maxDeg][degrees][edges]).  Returns 0,
+// 1 (cannot open) or 2 (short write).
+inline int writeGraphListsFile(const char* path, const std::vector<std::vector<uint32_t>>& adj) {
+    const uint32_t n = (uint32_t)adj.size();
+    uint32_t maxDeg = 0;
+    for (const auto& a : adj) maxDeg = std::max<uint32_t>(maxDeg, (uint32_t)a.size());
+    FILE* f = fopen(path, "wb");
+    if (!f) return 1;
+    const uint32_t hdr[2] = {n, maxDeg};
+    bool ok = fwrite(hdr, 4, 2, f) == 2;
+    for (uint32_t i = 0; ok && i < n; i++) {
+        const uint32_t d = (uint32_t)adj[i].size();
+        ok = fwrite(&d, 4, 1, f) == 1;
+    }
+    for (uint32_t i = 0; ok && i < n; i++)
+        ok = fwrite(adj[i].data(), 4, adj[i].size(), f) == adj[i].size();
+    ok = (fclose(f) == 0) && ok;
+    return ok ? 0 : 2;
+}
 
 // Greedy beam search used while building (exact visited set, not ParlayANN's lossy one).
 // Returns the expanded nodes with their distances in `visited`.

@@ -105,6 +105,23 @@ def build_hcnng(base, num_trees=30, leaf_size=1000, mst_deg=3, seed=7, nthreads=
         os.unlink(tmp)
 
 
+def build_hcnng_gpu(vectors, num_trees=30, leaf_size=1000, mst_deg=3, seed=7, path=None):
+    """HCNNG built on the device over the rows of a search.Vectors (pann_build_hcnng_gpu);
+    its graph file equals build_hcnng's for the same arguments."""
+    from ._lib import check, lib
+
+    def run(p):
+        check(lib().pann_build_hcnng_gpu(vectors.handle, num_trees, leaf_size, mst_deg, seed, p, None))
+    if path:
+        run(str(path).encode())
+        return formats.read_graph(path)
+    tmp = _via_file(run, ".graph")
+    try:
+        return formats.read_graph(tmp)
+    finally:
+        os.unlink(tmp)
+
+
 def build_pynndescent(base, K=40, num_trees=10, leaf_size=1000, alpha=1.2, delta=0.05,
                       max_degree=None, seed=7, nthreads=None, path=None):
     base = np.ascontiguousarray(base, dtype=np.int8)

@@ -79,3 +79,47 @@ def test_gpu_hnsw_build_equals_host(gpu, tmp_path, n, dim, M, efc, alpha, seed):
     finally:
         b.free()
     assert (tmp_path / "gpu.hnsw").read_bytes() == (tmp_path / "host.hnsw").read_bytes()
+
+
+HCNNG_CASES = [
+    # n, dim, num_trees, leaf_size, mst_deg, seed
+    (2, 16, 1, 2, 1, 1),
+    (500, 16, 3, 40, 3, 3),
+    (4000, 100, 4, 300, 3, 7),
+    (5000, 128, 3, 1000, 2, 5),
+    (6000, 37, 5, 1024, 4, 11),
+    (3000, 100, 2, 17, 3, 2),
+]
+
+
+@pytest.mark.parametrize("n,dim,trees,leaf,deg,seed", HCNNG_CASES)
+def test_gpu_hcnng_build_equals_host(gpu, tmp_path, n, dim, trees, leaf, deg, seed):
+    """The HCNNG graph file written by the device build equals the host builder's, byte
+    for byte: the same cluster-tree leaves, the same 16 nearest leaf mates, the same
+    degree-bounded spanning forests and the same order of every node's list.  Cases cover
+    leaves of 2 points, leaves of 17 to 1,024 points, rows with and without their norm
+    embedded (dim 100 vs 128), degree bounds 1 to 4 and duplicated points."""
+    from pann import builders, search
+    base = _data(n, dim, seed)
+    print(f"[build] hcnng n={n} dim={dim} trees={trees} leaf={leaf} deg={deg}", flush=True)
+    builders.build_hcnng(base, num_trees=trees, leaf_size=leaf, mst_deg=deg, seed=seed, nthreads=8,
+                         path=tmp_path / "host.graph")
+    b = search.Vectors(base)
+    try:
+        builders.build_hcnng_gpu(b, num_trees=trees, leaf_size=leaf, mst_deg=deg, seed=seed,
+                                 path=tmp_path / "gpu.graph")
+    finally:
+        b.free()
+    assert (tmp_path / "gpu.graph").read_bytes() == (tmp_path / "host.graph").read_bytes()
+
+
+def test_gpu_hcnng_rejects_bad_arguments(gpu):
+    from pann import PannError, builders, search
+    b = search.Vectors(np.zeros((4, 8), np.int8))
+    try:
+        with pytest.raises(PannError):
+            builders.build_hcnng_gpu(b, num_trees=1, leaf_size=1025, mst_deg=3)
+        with pytest.raises(PannError):
+            builders.build_hcnng_gpu(b, num_trees=1, leaf_size=100, mst_deg=5)
+    finally:
+        b.free()
