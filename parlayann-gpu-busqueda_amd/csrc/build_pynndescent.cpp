@@ -7,8 +7,9 @@
 // local joins (new x new and new x old neighbour pairs, reverse lists included) until an
 // iteration changes fewer than delta * n * K entries, then every node's K-NN list plus its
 // reverse edges is robust-pruned with alpha down to max_degree (as Vamana prunes).  The
-// pruned graph's components are then joined inside the cluster-tree leaves by their
-// closest pairs (step 4), which keeps the graph searchable from node 0 on clustered data.
+// pruned graph's components are then joined inside the cluster-tree leaves, each point
+// proposing its nearest leaf mate of another component (step 4), which keeps the graph
+// searchable from node 0 on clustered data.
 //
 // Each K-NN list keeps the K best (distance, id) pairs of everything proposed to it, so
 // the lists after an iteration do not depend on the order in which threads propose; the
@@ -84,13 +85,15 @@ void robustPrune(const int8_t* base, uint32_t dim, uint32_t p, std::vector<Nbr>&
 }  // namespace tools
 }  // namespace pann
 
+constexpr uint32_t kBridgeCap = 24;  // bridges per node at most (step 4)
+
 extern "C" int pann_build_pynndescent(const int8_t* base, uint32_t n, uint32_t dim, uint32_t K,
                                       uint32_t num_trees, uint32_t leaf_size, float alpha,
                                       float delta, uint32_t max_degree, uint64_t seed, int nthreads,
                                       const char* out_path) {
     using namespace pann::tools;
     if (!base || !out_path || n < 2 || dim == 0 || K == 0 || num_trees == 0 || leaf_size < 2 ||
-        max_degree == 0)
+        leaf_size > 1024 || max_degree == 0)
         return setError(1, "invalid argument");
     nthreads = std::max(1, nthreads);
     K = std::min(K, n - 1);
@@ -180,15 +183,14 @@ extern "C" int pann_build_pynndescent(const int8_t* base, uint32_t n, uint32_t d
     // 4. leaf bridges.  A K-NN graph of well-separated clusters falls apart into one
     // component per cluster, and a search that starts at node 0 never leaves its own.
     // The cluster trees of step 1 put pieces of neighbouring clusters into one leaf; in
-    // every such leaf the pruned graph's components are joined the way a minimum spanning
-    // tree joins them: the closest pair between each two components present, taken by
-    // Kruskal in (distance, endpoints) order over the leaf's components, each accepted
-    // pair added in both directions.  Each leaf's Kruskal is its own, so a cluster keeps
-    // the bridges of every tree, as HCNNG's per-leaf spanning trees do.  A node takes part
-    // in at most kBridgeCap bridges (pairs are drawn from the nodes with room left when
-    // the tree starts, and a pair whose endpoint filled up in this leaf is skipped), so
-    // rows stay within max_degree + kBridgeCap entries.
-    constexpr uint32_t kBridgeCap = 24;
+    // every such leaf each point proposes its nearest leaf mate of another component, and
+    // the proposals are taken by Kruskal over the leaf's components in (distance, leaf
+    // position) order -- the way HCNNG's per-leaf spanning trees join a leaf, over each
+    // point's nearest leaf mates -- each accepted pair added in both directions.  Each
+    // leaf's Kruskal is its own, so a cluster keeps the bridges of every tree.  A node
+    // takes part in at most kBridgeCap bridges: only nodes with room left when the tree
+    // starts propose or are proposed, and a pair whose endpoint filled up in this leaf is
+    // skipped, so rows stay within max_degree + kBridgeCap entries.
     std::vector<uint32_t> comp(n), bridges(n, 0);
     {
         std::vector<uint32_t> p(n);
@@ -212,57 +214,43 @@ extern "C" int pann_build_pynndescent(const int8_t* base, uint32_t n, uint32_t d
         parallelFor(leaves.size(), nthreads, [&](size_t li, int) {
             const uint32_t b = leaves[li].first, m = leaves[li].second - b;
             const uint32_t* L = ids.data() + b;
-            std::vector<uint32_t> cs(m);
-            for (uint32_t i = 0; i < m; i++) cs[i] = comp[L[i]];
-            std::vector<uint32_t> distinct(cs);
-            std::sort(distinct.begin(), distinct.end());
-            distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
-            const uint32_t C = (uint32_t)distinct.size();
-            if (C < 2) return;
-            for (auto& c : cs) c = (uint32_t)(std::lower_bound(distinct.begin(), distinct.end(), c) - distinct.begin());
-            // closest pair per component pair, (distance, smaller id, larger id) order
-            struct Pair {
-                int32_t d;
-                uint32_t x, y;
-                bool operator<(const Pair& o) const {
-                    return d != o.d ? d < o.d : x != o.x ? x < o.x : y < o.y;
+            // rep[i]: first leaf position of i's component; edges (d, lower, upper position)
+            std::vector<uint32_t> rep(m);
+            std::vector<uint64_t> edges;
+            for (uint32_t i = 0; i < m; i++) {
+                rep[i] = i;
+                uint64_t best = ~0ull;
+                const bool room = bridges[L[i]] < kBridgeCap;
+                for (uint32_t j = 0; j < m; j++) {
+                    if (comp[L[j]] == comp[L[i]]) {
+                        rep[i] = std::min(rep[i], j);
+                        continue;
+                    }
+                    if (!room || bridges[L[j]] >= kBridgeCap) continue;
+                    const uint64_t key = ((uint64_t)(uint32_t)distI8(vec(L[i]), vec(L[j]), dim) << 32) | j;
+                    best = std::min(best, key);
                 }
-            };
-            std::vector<Pair> best((size_t)C * C, Pair{INT32_MAX, 0xFFFFFFFFu, 0xFFFFFFFFu});
-            for (uint32_t i = 0; i < m; i++)
-                for (uint32_t j = i + 1; j < m; j++) {
-                    if (cs[i] == cs[j] || bridges[L[i]] >= kBridgeCap || bridges[L[j]] >= kBridgeCap) continue;
-                    const uint32_t x = std::min(L[i], L[j]), y = std::max(L[i], L[j]);
-                    const Pair c{distI8(vec(x), vec(y), dim), x, y};
-                    Pair& slot = best[(size_t)std::min(cs[i], cs[j]) * C + std::max(cs[i], cs[j])];
-                    if (c < slot) slot = c;
+                if (best != ~0ull) {
+                    const uint32_t j = (uint32_t)best;
+                    edges.push_back(((best >> 32) << 20) | ((uint64_t)std::min(i, j) << 10) | std::max(i, j));
                 }
-            std::vector<std::pair<Pair, std::pair<uint32_t, uint32_t>>> edges;
-            for (uint32_t a = 0; a < C; a++)
-                for (uint32_t c = a + 1; c < C; c++)
-                    if (best[(size_t)a * C + c].d != INT32_MAX) edges.push_back({best[(size_t)a * C + c], {a, c}});
-            std::sort(edges.begin(), edges.end(),
-                      [](const auto& u, const auto& v) { return u.first < v.first; });
-            std::vector<uint32_t> p(C);
-            std::iota(p.begin(), p.end(), 0u);
+            }
+            std::sort(edges.begin(), edges.end());
+            edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+            std::vector<uint32_t> p(rep), used(m, 0);
             auto find = [&](uint32_t x) {
                 while (p[x] != x) x = p[x] = p[p[x]];
                 return x;
             };
-            std::vector<uint32_t> used(m, 0);  // bridges taken in this leaf, by leaf index
-            auto room = [&](uint32_t x) {
-                const uint32_t k = (uint32_t)(std::find(L, L + m, x) - L);
-                return bridges[x] + used[k] < kBridgeCap ? k : m;
-            };
-            for (const auto& e : edges) {
-                const uint32_t ra = find(e.second.first), rb = find(e.second.second);
-                if (ra == rb) continue;
-                const uint32_t kx = room(e.first.x), ky = room(e.first.y);
-                if (kx == m || ky == m) continue;
-                p[ra] = rb;
-                used[kx]++;
-                used[ky]++;
-                found[li].push_back({e.first.x, e.first.y});
+            for (const uint64_t e : edges) {
+                const uint32_t x = (uint32_t)(e >> 10) & 1023u, y = (uint32_t)e & 1023u;
+                const uint32_t rx = find(x), ry = find(y);
+                if (rx == ry) continue;
+                if (bridges[L[x]] + used[x] >= kBridgeCap || bridges[L[y]] + used[y] >= kBridgeCap) continue;
+                p[rx] = ry;
+                used[x]++;
+                used[y]++;
+                found[li].push_back({L[x], L[y]});
             }
         }, 1);
         for (const auto& f : found)
