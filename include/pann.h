@@ -354,6 +354,23 @@ int pann_build_hnsw_gpu(const pann_vectors* base, uint32_t M, uint32_t efc, floa
 int pann_build_hcnng_gpu(const pann_vectors* base, uint32_t num_trees, uint32_t leaf_size,
                          uint32_t mst_deg, uint64_t seed, const char* out_path, void* stream);
 
+/* PyNNDescent graph over the rows of `base`, built on the device and written as a
+ * ParlayANN graph file: K-NN lists (K <= 64) from num_trees random cluster trees (exact
+ * nearest leaf mates, leaves of at most leaf_size points), NN-descent rounds (new and
+ * old neighbours, reverse lists of K/2, local joins) until a round changes fewer than
+ * delta * n * K entries (12 at most), every list plus its reverse edges robust-pruned with
+ * alpha to max_degree, the pruned graph's components joined inside the cluster-tree leaves
+ * (each point's nearest leaf mate of another component, Kruskal, 24 bridges per node at
+ * most), and nodes node 0 still cannot reach given one edge from their nearest reachable
+ * node.  The reference's GPU plan for PyNNDescent (README.md:354-356) and otherwise loads
+ * the prebuilt graph (leerGrafo, cuda/lectores.cpp:97-144).  The file equals the one
+ * pann_build_pynndescent (include/pann_tools.h) writes for the same arguments, byte for
+ * byte.  leaf_size <= 1024, max_degree <= 128; PANN_ERR_CAPACITY if a prune meets more
+ * than 2048 candidates.  Blocking. */
+int pann_build_pynndescent_gpu(const pann_vectors* base, uint32_t K, uint32_t num_trees,
+                               uint32_t leaf_size, float alpha, float delta, uint32_t max_degree,
+                               uint64_t seed, const char* out_path, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -847,6 +847,31 @@ int pann_build_hcnng_gpu(const pann_vectors* base, uint32_t num_trees, uint32_t 
     return r;
 }
 
+int pann_build_pynndescent_gpu(const pann_vectors* base, uint32_t K, uint32_t num_trees,
+                               uint32_t leaf_size, float alpha, float delta, uint32_t max_degree,
+                               uint64_t seed, const char* out_path, void* stream) {
+    if (!base || !out_path) return setError(PANN_ERR_ARG, "NULL argument");
+    if (base->n < 2) return setError(PANN_ERR_ARG, "fewer than 2 points");
+    if (K == 0 || K > 64) return setError(PANN_ERR_CAPACITY, "K=%u outside 1..64", K);
+    if (num_trees == 0) return setError(PANN_ERR_ARG, "num_trees must be positive");
+    if (leaf_size < 2 || leaf_size > 1024)
+        return setError(PANN_ERR_CAPACITY, "leaf_size=%u outside 2..1024", leaf_size);
+    if (max_degree == 0 || max_degree > 128)
+        return setError(PANN_ERR_CAPACITY, "max_degree=%u outside 1..128", max_degree);
+    if (alpha != alpha || delta != delta) return setError(PANN_ERR_ARG, "alpha or delta is NaN");
+    int cus = 0;
+    int r = deviceCUs(&cus);
+    if (r != PANN_OK) return r;
+    std::vector<std::vector<uint32_t>> adj;
+    uint32_t status = 0;
+    const hipError_t e = launchBuildPynndescent(base->dev, base->dim <= (uint32_t)kNormInRowMaxDim, base->n, K,
+                                                num_trees, leaf_size, alpha, delta, max_degree, seed, cus, adj,
+                                                &status, (hipStream_t)stream);
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "PyNNDescent build: %s", hipGetErrorString(e));
+    if (status) return setError(PANN_ERR_CAPACITY, "a robust prune met more than 2048 candidates");
+    return bridgeAndWrite(base, adj, out_path);
+}
+
 int pann_build_hnsw_gpu(const pann_vectors* base, uint32_t M, uint32_t efc, float ml, float alpha,
                         uint64_t seed, const char* out_path, void* stream) {
     if (!base || !out_path) return setError(PANN_ERR_ARG, "NULL argument");

@@ -312,6 +312,12 @@ hipError_t launchBuildVamana(const int8_t* vec, bool normsInRow, uint32_t n, uin
 hipError_t launchBuildHcnng(const int8_t* vec, bool normsInRow, uint32_t n, uint32_t numTrees,
                             uint32_t leafSize, uint32_t mstDeg, uint64_t seed,
                             std::vector<std::vector<uint32_t>>& adj, hipStream_t s);
+// PyNNDescent over the n device rows of vec: adj[i] = node i's list, before the final
+// bridging from node 0.  *status: 0, or 2 when a prune met more than 2048 candidates.
+hipError_t launchBuildPynndescent(const int8_t* vec, bool normsInRow, uint32_t n, uint32_t K,
+                                  uint32_t numTrees, uint32_t leafSize, float alpha, float delta,
+                                  uint32_t maxDeg, uint64_t seed, int cus,
+                                  std::vector<std::vector<uint32_t>>& adj, uint32_t* status, hipStream_t s);
 hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_t n,
                                const int8_t* queries, const int32_t* qnorm, uint32_t nq,
                                uint32_t k, bool normsInRow, uint32_t* outIds, int32_t* outDists,

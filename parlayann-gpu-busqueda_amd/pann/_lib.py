@@ -118,6 +118,8 @@ PANN_SIGNATURES = {
     "pann_build_vamana_gpu": (C.c_int, [_P, _u32, _u32, C.c_float, C.c_int, _u64, _P, _P, _P]),
     "pann_build_hnsw_gpu": (C.c_int, [_P, _u32, _u32, C.c_float, C.c_float, _u64, C.c_char_p, _P]),
     "pann_build_hcnng_gpu": (C.c_int, [_P, _u32, _u32, _u32, _u64, C.c_char_p, _P]),
+    "pann_build_pynndescent_gpu": (C.c_int, [_P, _u32, _u32, _u32, C.c_float, C.c_float, _u32, _u64,
+                                             C.c_char_p, _P]),
     "pann_batch_prepare_flat": (C.c_int, [_P, _P, _u32, C.POINTER(BatchConfig), _PP]),
     "pann_batch_prepare_hnsw": (C.c_int, [_P, _u32, C.POINTER(BatchConfig), _PP]),
     "pann_batch_set_queries": (C.c_int, [_P, _P, _P, _u32, _P]),

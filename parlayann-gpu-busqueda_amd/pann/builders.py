@@ -140,3 +140,24 @@ def build_pynndescent(base, K=40, num_trees=10, leaf_size=1000, alpha=1.2, delta
         return formats.read_graph(tmp)
     finally:
         os.unlink(tmp)
+
+
+def build_pynndescent_gpu(vectors, K=40, num_trees=10, leaf_size=1000, alpha=1.2, delta=0.05,
+                          max_degree=None, seed=7, path=None):
+    """PyNNDescent built on the device over the rows of a search.Vectors
+    (pann_build_pynndescent_gpu); its graph file equals build_pynndescent's for the same
+    arguments."""
+    from ._lib import check, lib
+    md = int(max_degree or K)
+
+    def run(p):
+        check(lib().pann_build_pynndescent_gpu(vectors.handle, K, num_trees, leaf_size, float(alpha),
+                                               float(delta), md, seed, p, None))
+    if path:
+        run(str(path).encode())
+        return formats.read_graph(path)
+    tmp = _via_file(run, ".graph")
+    try:
+        return formats.read_graph(tmp)
+    finally:
+        os.unlink(tmp)

@@ -123,3 +123,53 @@ def test_gpu_hcnng_rejects_bad_arguments(gpu):
             builders.build_hcnng_gpu(b, num_trees=1, leaf_size=100, mst_deg=5)
     finally:
         b.free()
+
+
+PYNN_CASES = [
+    # n, dim, K, num_trees, leaf_size, alpha, delta, max_degree, seed, ncomp
+    (2, 16, 4, 1, 2, 1.2, 0.05, 4, 1, 1),
+    (600, 16, 8, 3, 50, 1.2, 0.05, 8, 3, 3),
+    (4000, 100, 16, 4, 300, 1.2, 0.05, 16, 7, 20),
+    (5000, 128, 40, 3, 1000, 1.2, 0.05, 40, 5, 25),
+    (6000, 37, 24, 5, 1024, 1.1, 0.02, 32, 11, 30),
+    (3000, 100, 40, 2, 200, 1.3, 0.1, 60, 2, 15),
+]
+
+
+@pytest.mark.parametrize("n,dim,K,trees,leaf,alpha,delta,md,seed,ncomp", PYNN_CASES)
+def test_gpu_pynndescent_build_equals_host(gpu, tmp_path, n, dim, K, trees, leaf, alpha, delta, md, seed,
+                                           ncomp):
+    """The PyNNDescent graph file written by the device build equals the host builder's,
+    byte for byte: the same cluster-tree K-NN lists, the same NN-descent rounds (sampled
+    new/old lists, reverse lists, joins, stop rule), the same robust prune of lists plus
+    reverse edges, and the same leaf bridges between components.  Cases cover K from 4 to
+    40, lists shorter than K (leaves of 2 points), rows with and without their norm
+    embedded, clustered data with many components, and duplicated points."""
+    from pann import builders, datagen, search
+    base, _ = datagen.gmm_dataset(n, 1, dim=dim, ncomp=ncomp, seeds=(seed, seed + 1, seed + 2))
+    if n >= 100:
+        base[n // 2: n // 2 + 10] = base[:10]
+    print(f"[build] pynndescent n={n} dim={dim} K={K} trees={trees} leaf={leaf} md={md}", flush=True)
+    builders.build_pynndescent(base, K=K, num_trees=trees, leaf_size=leaf, alpha=alpha, delta=delta,
+                               max_degree=md, seed=seed, nthreads=8, path=tmp_path / "host.graph")
+    b = search.Vectors(base)
+    try:
+        builders.build_pynndescent_gpu(b, K=K, num_trees=trees, leaf_size=leaf, alpha=alpha, delta=delta,
+                                       max_degree=md, seed=seed, path=tmp_path / "gpu.graph")
+    finally:
+        b.free()
+    assert (tmp_path / "gpu.graph").read_bytes() == (tmp_path / "host.graph").read_bytes()
+
+
+def test_gpu_pynndescent_rejects_bad_arguments(gpu):
+    from pann import PannError, builders, search
+    b = search.Vectors(np.zeros((4, 8), np.int8))
+    try:
+        with pytest.raises(PannError):
+            builders.build_pynndescent_gpu(b, K=65)
+        with pytest.raises(PannError):
+            builders.build_pynndescent_gpu(b, K=8, leaf_size=2000)
+        with pytest.raises(PannError):
+            builders.build_pynndescent_gpu(b, K=8, max_degree=129)
+    finally:
+        b.free()

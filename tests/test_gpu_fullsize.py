@@ -4,7 +4,7 @@
   * 1,000,000 x 100 int8 GMM(1,024), 29,316 queries, four graphs (Vamana R=64 L=128
     alpha=1.2; HCNNG 30 trees, leaf 1,000, MST degree 3; PyNNDescent K=40, 10 trees; HNSW
     M=32 efC=128), the 18 Ls values 10..500: every query at every Ls, plain launches;
-  * the same Vamana, HCNNG and HNSW graphs built on the device, against the host builders';
+  * the same four graphs built on the device, against the host builders';
   * 65,536 queries with per-query Ls log-uniform in [10, 400], k in {1, 10, 100}, visit
     limit 2 Ls, as one batched launch on the 1M Vamana graph: every query;
   * the serving loop on the resident 1M Vamana and HNSW indexes under Poisson arrivals
@@ -140,6 +140,20 @@ def test_1m_gpu_hcnng_build(data1m, capsys, tmp_path):
     t = time.time()
     g = builders.build_hcnng_gpu(b, num_trees=30, leaf_size=1000, mst_deg=3, seed=7)
     _say(capsys, f"GPU HCNNG build of 1M in {time.time() - t:.2f}s")
+    assert g.max_degree == host.max_degree
+    assert np.array_equal(g.degrees, host.degrees)
+    assert np.array_equal(g.edges, host.edges)
+
+
+def test_1m_gpu_pynndescent_build(data1m, capsys):
+    """The 1M PyNNDescent graph (K=40, 10 trees) built on the device equals the host
+    builder's, row for row."""
+    from pann import builders
+    base, queries, b, q = data1m
+    host, _ = _graph("pynndescent", base, b, capsys)
+    t = time.time()
+    g = builders.build_pynndescent_gpu(b, K=40, num_trees=10, seed=7)
+    _say(capsys, f"GPU PyNNDescent build of 1M in {time.time() - t:.2f}s")
     assert g.max_degree == host.max_degree
     assert np.array_equal(g.degrees, host.degrees)
     assert np.array_equal(g.edges, host.edges)
