@@ -973,6 +973,16 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
 
         // visit u: its flag, and its expansion count (64 probe slots read at once)
         if (lane == u) F |= 1u;
+        // Prefetch the row of the likely next node: the first unvisited frontier entry after
+        // u, which is the next node unless this hop brings a closer candidate.  The load has
+        // the whole hop to arrive; a wrong guess costs one row, and the next hop loads its own.
+        if constexpr (PREFETCH) {
+            const uint64_t unv = ballot(lane < fsize && !(F & 1u));
+            predId = unv ? keyId(readlane64(F, __ffsll((unsigned long long)unv) - 1)) : kEmpty;
+            aNext = kEmpty;
+            if (predId != kEmpty && (uint32_t)lane < G)
+                aNext = *rowAt<NARROW, uint32_t>(L.adj, predId, 4 * G, 4 * (uint32_t)lane);
+        }
         bool overflow = false;
         {
             const uint32_t h0 = visitedSlot(uid, logCap);
@@ -1071,35 +1081,6 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
         const int d = qq + transposeReduce8(part8, lane);  // distance of entry `lane`
         const uint64_t C = (keep && d < cutoff) ? tagKey(d, a, kCand | cnt) : kNoKey;
         P.mark(kProfDist);
-
-        // Prefetch the row of the likely next node: the better (by distance) of the next
-        // unvisited frontier entry and the closest candidate never expanded.  A wrong guess
-        // costs one row; the next hop then loads its own.
-        if constexpr (PREFETCH) {
-            const uint64_t unv = ballot(lane < fsize && !(F & 1u));
-            int dF = 0x7FFFFFFF;
-            predId = kEmpty;
-            if (unv) {
-                const uint64_t f = readlane64(F, __ffsll((unsigned long long)unv) - 1);
-                dF = keyDist(f);
-                predId = keyId(f);
-            }
-            const bool fresh = C != kNoKey && (C & 0x7Fu) == 0;
-            uint32_t dd = fresh ? (uint32_t)d : 0xFFFFFFFFu;
-            dd = min(dd, dppMov<0xB1>(dd));   // quad
-            dd = min(dd, dppMov<0x4E>(dd));
-            dd = min(dd, dppMov<0x124>(dd));  // row: rotate by 4, then by 8
-            dd = min(dd, dppMov<0x128>(dd));
-            const uint32_t m = min(min((uint32_t)readlane((int)dd, 0), (uint32_t)readlane((int)dd, 16)),
-                                   min((uint32_t)readlane((int)dd, 32), (uint32_t)readlane((int)dd, 48)));
-            if (m < (uint32_t)dF) {
-                const uint64_t w = ballot(fresh && (uint32_t)d == m);
-                predId = (uint32_t)readlane((int)a, __ffsll((unsigned long long)w) - 1);
-            }
-            aNext = kEmpty;
-            if (predId != kEmpty && (uint32_t)lane < G)
-                aNext = *rowAt<NARROW, uint32_t>(L.adj, predId, 4 * G, 4 * (uint32_t)lane);
-        }
 
         const uint64_t candM = ballot(C != kNoKey);
         const int nc = __popcll(candM);
