@@ -210,8 +210,10 @@ typedef struct pann_batch_config {
     int32_t lds_filter_bits; /* seen filters up to 2^bits slots live in LDS, 2..12 (0: 10) */
     uint32_t visited_slots;  /* visited / is_inw hash slots per query (LDS, or HBM from 2048 /
                                 4096 slots), power of two >= 64 (0: derived) */
-    uint32_t prefetch;       /* next-row prefetch depth, 0 or 1: while a hop merges, load the
-                                row of the predicted next node (flat graphs, max_Ls <= 64) */
+    uint32_t prefetch;       /* next-row prefetch depth, 0, 1 or 2: at the start of a hop, load
+                                the rows of the first (and second) unvisited frontier entries
+                                after the expanded node, the likely next nodes (flat graphs,
+                                max_Ls <= 64) */
     uint32_t waves_per_query; /* wavefronts sharing one query, 1, 2 or 4 (0: 1), in the flat
                                  launches of the general form (beams above 128 or rows above
                                  64 entries): the leader filters and merges the candidates,

@@ -915,7 +915,7 @@ __device__ __forceinline__ UnionMarks unionMarks(uint64_t& x, uint64_t prev, int
     return m;
 }
 
-template <bool NARROW, bool PREFETCH, bool NORMS>
+template <bool NARROW, int PREFETCH, bool NORMS>
 __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, const QueryCfg& c,
                            Prof& P) {
     const int lane = laneId();
@@ -959,8 +959,8 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
     int comps = 1, edges = 0, hops = 0, flags = 0;
     uint32_t distinct = 0;
     const uint32_t G = L.G;
-    uint32_t predId = kEmpty;  // node whose row was prefetched into aNext
-    uint32_t aNext = kEmpty;
+    uint32_t predId = kEmpty, predId2 = kEmpty;  // nodes whose rows were prefetched into
+    uint32_t aNext = kEmpty, aNext2 = kEmpty;    // aNext (and, depth 2, aNext2)
     __builtin_amdgcn_wave_barrier();
     P.mark(kProfInit);
     P.count(kProfQueries);
@@ -968,20 +968,30 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
     while (u >= 0 && visits < c.limit) {
         const uint32_t uid = keyId(readlane64(F, u));
         uint32_t a = aNext;
-        if (!PREFETCH || uid != predId)
+        if (PREFETCH >= 2 && uid != predId && uid == predId2) a = aNext2;
+        else if (PREFETCH == 0 || uid != predId)
             a = (uint32_t)lane < G ? *rowAt<NARROW, uint32_t>(L.adj, uid, 4 * G, 4 * (uint32_t)lane) : kEmpty;
 
         // visit u: its flag, and its expansion count (64 probe slots read at once)
         if (lane == u) F |= 1u;
         // Prefetch the row of the likely next node: the first unvisited frontier entry after
-        // u, which is the next node unless this hop brings a closer candidate.  The load has
-        // the whole hop to arrive; a wrong guess costs one row, and the next hop loads its own.
-        if constexpr (PREFETCH) {
-            const uint64_t unv = ballot(lane < fsize && !(F & 1u));
+        // u, which is the next node unless this hop brings a closer candidate (depth 2: the
+        // second one too, the next node when the first is displaced by one candidate...).  The
+        // loads have the whole hop to arrive; a wrong guess costs one row, and the next hop
+        // loads its own.
+        if constexpr (PREFETCH >= 1) {
+            uint64_t unv = ballot(lane < fsize && !(F & 1u));
             predId = unv ? keyId(readlane64(F, __ffsll((unsigned long long)unv) - 1)) : kEmpty;
             aNext = kEmpty;
             if (predId != kEmpty && (uint32_t)lane < G)
                 aNext = *rowAt<NARROW, uint32_t>(L.adj, predId, 4 * G, 4 * (uint32_t)lane);
+            if constexpr (PREFETCH >= 2) {
+                unv &= unv - 1;
+                predId2 = unv ? keyId(readlane64(F, __ffsll((unsigned long long)unv) - 1)) : kEmpty;
+                aNext2 = kEmpty;
+                if (predId2 != kEmpty && (uint32_t)lane < G)
+                    aNext2 = *rowAt<NARROW, uint32_t>(L.adj, predId2, 4 * G, 4 * (uint32_t)lane);
+            }
         }
         bool overflow = false;
         {
@@ -1180,7 +1190,8 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
         const uint64_t nv = ballot(lane < fsize && !(F & 1u));
         u = nv ? __ffsll((unsigned long long)nv) - 1 : -1;
 #ifdef PANN_PROFILE
-        if (u >= 0 && keyId(readlane64(F, u)) == predId) P.count(kProfPredFC);
+        if (u >= 0 && (keyId(readlane64(F, u)) == predId || keyId(readlane64(F, u)) == predId2))
+            P.count(kProfPredFC);
 #endif
         P.mark(kProfNext);
     }
@@ -1548,7 +1559,13 @@ __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
         }
         makeUniform(c);
         const int fb = flatFilterBits(c.Ls);
-        if constexpr (FAST >= 9) {
+        if constexpr (FAST >= 13) {
+            if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride) {
+                rejectQuery(L, q, lane);
+            } else {
+                searchFast<((FAST - 13) & 1) != 0, 2, ((FAST - 13) & 2) != 0>(L, lds, q, c, P);
+            }
+        } else if constexpr (FAST >= 9) {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride || fb > L.gbitsMax) {
                 rejectQuery(L, q, lane);
             } else {
@@ -1558,15 +1575,15 @@ __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride) {
                 rejectQuery(L, q, lane);
             } else {
-                searchFast<((FAST - 1) & 1) != 0, ((FAST - 1) & 2) != 0, ((FAST - 1) & 4) != 0>(L, lds, q, c, P);
+                searchFast<((FAST - 1) & 1) != 0, ((FAST - 1) & 2) != 0 ? 1 : 0, ((FAST - 1) & 4) != 0>(L, lds, q, c, P);
             }
         } else {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride ||
                 (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
                 rejectQuery(L, q, lane);
             } else if (!VG && MAXCH == 1 && c.Ls <= kFastLsMax && L.fbitsLds >= kFastFilterBits) {
-                if (L.normsInRow) searchFast<false, false, true>(L, lds, q, c, P);
-                else searchFast<false, false, false>(L, lds, q, c, P);
+                if (L.normsInRow) searchFast<false, 0, true>(L, lds, q, c, P);
+                else searchFast<false, 0, false>(L, lds, q, c, P);
             } else if (fb <= L.fbitsLds) {
                 searchOne<MAXCH, false, VG>(L, lds, q, c, nullptr, gvk, gvc, P);
             } else {
@@ -1734,6 +1751,17 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
     }
     if (L.fast) {
         if (maxch != 1) return hipErrorInvalidValue;
+        if (L.fast >= 13) {
+            if (L.LsMax > kFastLsMax) return hipErrorInvalidValue;
+            switch (L.fast + (L.normsInRow ? 2 : 0)) {
+                case 13: hipLaunchKernelGGL((beamFlatFastKernel<13>), grid, block, lds, stream, L); break;
+                case 14: hipLaunchKernelGGL((beamFlatFastKernel<14>), grid, block, lds, stream, L); break;
+                case 15: hipLaunchKernelGGL((beamFlatFastKernel<15>), grid, block, lds, stream, L); break;
+                case 16: hipLaunchKernelGGL((beamFlatFastKernel<16>), grid, block, lds, stream, L); break;
+                default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        }
         if (L.fast >= 9) {
             if (L.LsMax > kFast2LsMax || L.gfilt == nullptr) return hipErrorInvalidValue;
             switch (L.fast + (L.normsInRow ? 2 : 0)) {
@@ -1789,6 +1817,11 @@ hipError_t beamFlatOccupancy(int maxch, int fast, int wavesPerBlock, size_t ldsB
             case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatWideKernel<4>, threads, ldsBytes);
             default: return hipErrorInvalidValue;
         }
+    }
+    if (fast >= 13) {
+        if (maxch != 1) return hipErrorInvalidValue;
+        return fast == 13 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFastKernel<13>, threads, ldsBytes)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFastKernel<14>, threads, ldsBytes);
     }
     if (fast >= 9) {
         if (maxch != 1) return hipErrorInvalidValue;

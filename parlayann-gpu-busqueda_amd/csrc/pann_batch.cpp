@@ -81,7 +81,7 @@ int checkBatchConfig(const pann_batch_config* cfg) {
         return setError(PANN_ERR_ARG, "visited_slots must be 0 or a power of two >= 64");
     if (cfg->waves_per_query > 4 || cfg->waves_per_query == 3)
         return setError(PANN_ERR_ARG, "waves_per_query must be 0, 1, 2 or 4");
-    if (cfg->prefetch > 1) return setError(PANN_ERR_ARG, "prefetch must be 0 or 1");
+    if (cfg->prefetch > 2) return setError(PANN_ERR_ARG, "prefetch must be 0, 1 or 2");
     return PANN_OK;
 }
 
@@ -161,7 +161,8 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int*
     const int mode = fastMode(b->graph->n, b->graph->G);
     P->LsLo = LsLo;
     P->LsHi = LsHi;
-    P->fastKind = fast ? mode + (c.prefetch ? 2 : 0) : fast2 ? 8 + mode : hbmVisited ? -1 : 0;
+    P->fastKind = fast ? (c.prefetch == 2 ? 12 + mode : mode + (c.prefetch ? 2 : 0))
+                       : fast2 ? 8 + mode : hbmVisited ? -1 : 0;
     // several wavefronts per query: the general form only
     const int wpq = (!fast && !fast2 && c.waves_per_query > 1) ? (int)c.waves_per_query : 1;
     FlatLds lay = fast ? flatFastLdsLayout(capV)
@@ -605,7 +606,7 @@ int pann_batch_autotune(const pann_vectors* base, const pann_graph* g, const pan
             for (uint32_t vs : {derived, derived / 2, derived * 2}) {
                 if (vs < (h ? 512u : 256u) || vs > 16384) continue;
                 if (!h && vs < 2u * (uint32_t)max_Ls + 64) continue;
-                for (uint32_t pf : {0u, 1u}) {
+                for (uint32_t pf : {0u, 1u, 2u}) {
                     if (pf && (h || max_Ls > kFastLsMax)) continue;
                     // several wavefronts per query: flat graphs whose top class is the
                     // general form (beams above 128 or rows above 64)

@@ -159,6 +159,7 @@ def test_autotune_returns_clean_config(gpu, small):
 
 
 @pytest.mark.parametrize("prefetch,wide,max_Ls", [(0, False, 64), (1, False, 64), (1, True, 64),
+                                                  (2, False, 64), (2, True, 64),
                                                   (0, False, 128), (0, True, 128)])
 def test_register_resident_batch_forms(gpu, w1, monkeypatch, prefetch, wide, max_Ls):
     """Batches with max_Ls <= 64 (one frontier key per lane) or <= 128 (two keys per lane,
