@@ -409,16 +409,17 @@ def batched_vs_plain(b, g, rows, params, steps, tune_sample=8192):
     }
 
 
-def serving(b, g, queries, gt_ids, gt_d, rates, duration, hnsw=None):
+def serving(b, g, queries, gt_ids, gt_d, rates, duration, hnsw=None, max_batch=16384):
     """Poisson arrivals on the resident index (flat graph g, or the HNSW index): requests
     of 1..4096 queries, per-query Ls log-uniform in [10, 300], k in {10, 100}; p50/p99
     request latency (arrival to results in host memory), sustained QPS and recall@10 of
-    the served queries."""
+    the served queries.  max_batch 0: micro-batches sized by the LDS budget (one round of
+    resident wavefronts)."""
     from pann import recall, search
     if hnsw is not None:
-        srv = search.Server(slots=65536, max_batch=16384, streams=3, max_Ls=300, max_k=100, hnsw=hnsw)
+        srv = search.Server(slots=65536, max_batch=max_batch, streams=3, max_Ls=300, max_k=100, hnsw=hnsw)
     else:
-        srv = search.Server(slots=65536, max_batch=16384, streams=3, max_Ls=300, max_k=100, base=b,
+        srv = search.Server(slots=65536, max_batch=max_batch, streams=3, max_Ls=300, max_k=100, base=b,
                             graph=g)
     out = []
     srv.run_poisson(queries, 50000, 0.3, seed=99)  # warm-up
@@ -656,6 +657,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-rates", default="50000,200000,500000,1000000,2000000")
     ap.add_argument("--serving-seconds", type=float, default=2.0)
+    ap.add_argument("--serving-max-batch", type=int, default=16384,
+                    help="queries per serving micro-batch (0: the LDS budget, one round of resident wavefronts)")
     ap.add_argument("--no-serving-hnsw", action="store_true")
     ap.add_argument("--serving-hnsw-rates", default="50000,200000,500000,1000000,2000000")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
@@ -782,17 +785,17 @@ def main():
     serve = None
     if not args.no_serving:
         serve = serving(b, g, queries, gt_ids, gt_df, [float(x) for x in args.serving_rates.split(",")],
-                        args.serving_seconds)
+                        args.serving_seconds, max_batch=args.serving_max_batch)
     serve_hnsw = None
     if not args.no_serving and not args.no_serving_hnsw:
         from pann import builders
         t = time.time()
-        hg = builders.build_hnsw(base, M=32, efc=128, seed=7, nthreads=min(16, host_cores()))
-        log(f"[bench] HNSW M=32 efC=128 built in {time.time() - t:.1f}s for the serving loop")
+        hg = builders.build_hnsw_gpu(b, M=32, efc=128, seed=7)
+        log(f"[bench] HNSW M=32 efC=128 built on the GPU in {time.time() - t:.1f}s for the serving loop")
         hidx = search.HnswIndex(hg, b)
         serve_hnsw = serving(b, None, queries, gt_ids, gt_df,
                              [float(x) for x in args.serving_hnsw_rates.split(",")], args.serving_seconds,
-                             hnsw=hidx)
+                             hnsw=hidx, max_batch=args.serving_max_batch)
         hidx.free()
 
     cpu = None

@@ -267,7 +267,9 @@ typedef struct pann_server pann_server;
 
 typedef struct pann_server_config {
     uint32_t slots;           /* queries resident on the device at once (HBM budget) */
-    uint32_t max_batch;       /* queries per micro-batch launch */
+    uint32_t max_batch;       /* queries per micro-batch launch; 0: the LDS budget, i.e. the
+                                 wavefronts the CUs keep resident at once for the widest
+                                 query class (one round: every query starts at once) */
     int32_t streams;          /* micro-batches in flight at once */
     int32_t max_Ls;           /* per-query limits (LDS sizing) */
     int32_t max_k;

@@ -4,7 +4,8 @@
 // dispatcher thread owns the device work: it keeps `streams` lanes, each a batch of the
 // persistent form with pinned host staging.  Whenever a lane is idle it packs pending
 // queries (oldest request first, splitting requests across micro-batches as needed) up
-// to max_batch and to the device slot budget, and issues H2D copy, persistent launch and
+// to max_batch (by default the wavefronts the LDS budget keeps resident) and to the
+// device slot budget (HBM for query rows and results), and issues H2D copy, persistent launch and
 // D2H copy on the lane's stream, followed by an event.  Finished lanes hand their results
 // back to the requests.  Because each lane runs on its own stream, a micro-batch started
 // for newly arrived requests runs on the device alongside the ones already there.
@@ -207,11 +208,12 @@ int pann_server_create(const pann_vectors* base, const pann_graph* g, const pann
     if (!out) return setError(PANN_ERR_ARG, "out is NULL");
     *out = nullptr;
     if (!cfg || (!g && !h) || (g && !base)) return setError(PANN_ERR_ARG, "invalid arguments");
-    if (cfg->slots == 0 || cfg->max_batch == 0 || cfg->streams < 1 || cfg->streams > 8)
-        return setError(PANN_ERR_ARG, "slots, max_batch > 0 and 1 <= streams <= 8 required");
+    if (cfg->slots == 0 || cfg->streams < 1 || cfg->streams > 8)
+        return setError(PANN_ERR_ARG, "slots > 0 and 1 <= streams <= 8 required");
     auto s = std::make_unique<pann_server>();
     s->cfg = *cfg;
-    s->cfg.max_batch = std::min(cfg->max_batch, cfg->slots);
+    // max_batch 0: the LDS budget decides (below); lanes are sized for all the slots
+    s->cfg.max_batch = cfg->max_batch ? std::min(cfg->max_batch, cfg->slots) : cfg->slots;
     s->dim = h ? h->dim : base->dim;
     HIP_TRY(hipGetDevice(&s->device));
     const pann_batch_config bc{cfg->max_Ls, cfg->max_k, cfg->waves_per_block, cfg->lds_filter_bits,
@@ -243,6 +245,16 @@ int pann_server_create(const pann_vectors* base, const pann_graph* g, const pann
             hipHostMalloc((void**)&ln.hDists, mb * K * 4, 0) != hipSuccess ||
             hipHostMalloc((void**)&ln.hFlags, mb * 4, 0) != hipSuccess)
             return fail(setError(PANN_ERR_HIP, "allocating serving lanes"));
+    }
+    if (cfg->max_batch == 0) {
+        // Admission by the LDS budget: a micro-batch takes at most the wavefronts that the
+        // CUs' LDS (and registers) keep resident at once for the widest query class, so
+        // every query of it starts when the launch does and it takes as long as its
+        // slowest query, not several rounds of them.
+        uint32_t resident = 0;
+        const int rc = pann_batch_info(s->lanes[0].batch, nullptr, &resident);
+        if (rc != PANN_OK) return fail(rc);
+        s->cfg.max_batch = std::max<uint32_t>(1, std::min(s->cfg.max_batch, resident));
     }
     pann_server* raw = s.release();
     raw->worker = std::thread([raw] { raw->loop(); });
