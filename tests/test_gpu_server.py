@@ -88,3 +88,24 @@ def test_poisson_load(gpu, small):
     assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
     assert len(rows) == min(1000, st["queries"]) and (rows < len(queries)).all()
     srv.free(), g.free(), b.free()
+
+
+def test_lds_budget_micro_batches(gpu, small):
+    """max_batch 0 sizes micro-batches by the LDS budget (the batch's resident wavefronts);
+    served results still equal the oracle's."""
+    from pann import search
+    base, queries, graph = small
+    rng = np.random.default_rng(12)
+    reqs = _requests(12, queries, len(base), rng)
+    b, g = search.Vectors(base), search.FlatGraph(graph)
+    srv = search.Server(slots=2048, max_batch=0, streams=2, max_Ls=150, max_k=100, base=b, graph=g)
+    try:
+        tickets = [srv.submit(*r) for r in reqs]
+        for (rows, params), t in zip(reqs, tickets):
+            ids, d, f, _ = srv.wait(t)
+            o = _oracle(base, graph, rows, params)
+            for i, k in enumerate(params["k"]):
+                assert np.array_equal(ids[i, :k], o.ids[i, :k]) and np.array_equal(d[i, :k], o.dists[i, :k])
+            assert np.array_equal(f, o.flags)
+    finally:
+        srv.free(), g.free(), b.free()
