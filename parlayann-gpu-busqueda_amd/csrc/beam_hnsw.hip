@@ -430,14 +430,15 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
     __builtin_amdgcn_wave_barrier();
 }
 
-// On rows of at most 64 entries, two-register lists are held to 128 VGPRs (four wavefronts
-// per SIMD; a few spills) and five- to seven-register lists to 168 (three): the hop is a
-// chain of dependent memory round trips, and ef=100 / ef=256 / ef=350 on the 1M HNSW run
-// 1.17x / 1.3x / 1.2x faster than with the registers the compiler would take.  Eight-
-// register lists spill too much at 168 (ef=500: 52.3 ms held, 47.7 ms not).
+// On rows of at most 64 entries, one- and two-register lists are held to 128 VGPRs (four
+// wavefronts per SIMD; a few spills) and five- to seven-register lists to 168 (three): the
+// hop is a chain of dependent memory round trips, and ef=25 / ef=100 / ef=256 / ef=350 on
+// the 1M HNSW run 1.11x / 1.17x / 1.3x / 1.2x faster than with the registers the compiler
+// would take (one-register lists held to 96 VGPRs run 1.15x slower).  Eight-register lists
+// spill too much at 168 (ef=500: 52.3 ms held, 47.7 ms not).
 template <int WR, int MAXCH, bool IG>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(MAXCH == 1 && WR == 2 ? 4 : MAXCH == 1 && WR >= 5 && WR <= 7 ? 3 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(MAXCH == 1 && WR <= 2 ? 4 : MAXCH == 1 && WR >= 5 && WR <= 7 ? 3 : 1, 8)))
 void beamHnswKernel(HnswLaunch L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = laneId();
