@@ -651,12 +651,37 @@ hipError_t launchThreshold(const GemmArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Slices of `blocks` giving about `wantJobs` workgroups over qTiles query tiles, with at
-// least 64 blocks (1,024 rows) per slice.
-uint32_t pickSlices(uint32_t qTiles, uint32_t blocks, uint32_t wantJobs, uint32_t maxSlices) {
-    uint32_t s = (wantJobs + qTiles - 1) / qTiles;
-    s = std::min(s, std::max(1u, blocks / 64));
-    return std::max(1u, std::min(s, maxSlices));
+// Workgroups of `kernel` (256 threads) that all CUs keep resident at once.
+template <class Kernel>
+uint32_t residentBlocks(Kernel kernel) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0) != hipSuccess || per < 1 || cus < 1)
+        return 768;
+    return (uint32_t)(per * cus);
+}
+
+// Slices of `blocks` for qTiles query tiles (at most maxSlices, at least minBlocks blocks
+// per slice): the qTiles * slices workgroups should fill whole rounds of the `resident`
+// ones, since a round that is only partly filled takes as long as a full one (805
+// workgroups over 768 resident ran the 1M filter pass in two rounds).  The slice count with
+// the largest busy fraction wins, the fewest slices on ties.
+uint32_t pickSlices(uint32_t qTiles, uint32_t blocks, uint32_t resident, uint32_t maxSlices,
+                    uint32_t minBlocks) {
+    const uint32_t sMax = std::max(1u, std::min(maxSlices, blocks / std::max(1u, minBlocks)));
+    uint32_t best = 1;
+    double bestFill = -1;
+    for (uint32_t sl = 1; sl <= sMax; sl++) {
+        const uint64_t jobs = (uint64_t)qTiles * sl;
+        const uint64_t rounds = (jobs + resident - 1) / resident;
+        const double fill = (double)jobs / (double)(rounds * resident);
+        if (fill > bestFill + 1e-9) {
+            best = sl;
+            bestFill = fill;
+        }
+    }
+    return best;
 }
 
 struct DevBuf {
@@ -682,7 +707,9 @@ hipError_t thresholds(GemmArgs a, int L, uint32_t stride, int32_t* thr, hipStrea
     if (e != hipSuccess) return e;
     a.scanBlocks = sampleBlocks;
     a.stride = stride;
-    a.slices = pickSlices(a.qTiles, sampleBlocks, 768, 64);
+    const uint32_t res = L == 1 ? residentBlocks(bfThresholdKernel<1>) : L == 2 ? residentBlocks(bfThresholdKernel<2>)
+                       : L == 4 ? residentBlocks(bfThresholdKernel<4>) : residentBlocks(bfThresholdKernel<8>);
+    a.slices = pickSlices(a.qTiles, sampleBlocks, res, 64, 512);
     a.jobs = a.qTiles * a.slices;
     a.accmax = static_cast<int32_t*>(accmax.p);
     switch (L) {
@@ -780,7 +807,8 @@ hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_
     a.thr = static_cast<const int32_t*>(thr.p);
     a.scanBlocks = a.nBlocks;
     a.stride = 1;
-    a.slices = pickSlices(a.qTiles, a.nBlocks, 768, 32);
+    const uint32_t resFilter = residentBlocks(bfFilterKernel);
+    a.slices = pickSlices(a.qTiles, a.nBlocks, resFilter, 32, 512);
     a.capS = std::max(16u, perQuery / a.slices);
     std::vector<uint32_t> counts, retry;
     e = filterSelect(a, nullptr, (uint64_t)nq * a.slices * a.capS, k, outIds, outDists, counts, retry, s);
@@ -820,7 +848,7 @@ hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_
         r.qTiles = (r.nsel + kQueriesPerBlock - 1) / kQueriesPerBlock;
         r.scanBlocks = r.nBlocks;
         r.stride = 1;
-        r.slices = pickSlices(r.qTiles, r.nBlocks, 768, kMaxSlices);
+        r.slices = pickSlices(r.qTiles, r.nBlocks, resFilter, kMaxSlices, 64);
         r.capS = 0;  // count only
         std::vector<uint32_t> c1, r1;
         e = filterSelect(r, nullptr, 1, k, outIds, outDists, c1, r1, s);
