@@ -689,21 +689,36 @@ struct DevBuf {
     ~DevBuf() { (void)hipFree(p); }
 };
 
-hipError_t upload(DevBuf& d, const void* host, size_t bytes, hipStream_t s) {
-    hipError_t e = hipMalloc(&d.p, std::max<size_t>(bytes, 8));
-    if (e == hipSuccess && bytes) e = hipMemcpyAsync(d.p, host, bytes, hipMemcpyHostToDevice, s);
-    return e;
-}
+// The device memory of one call: one allocation cut into 256-byte aligned pieces, taken
+// in the order they were added.  A call allocates once and frees once (about a millisecond
+// each for the half-gigabyte candidate lists of 29,316 queries over 1M rows), instead of
+// once per pass and buffer.
+class Arena {
+public:
+    void add(size_t bytes) { total_ += pad(bytes); }
+    hipError_t allocate() { return hipMalloc(&mem_.p, std::max<size_t>(total_, 256)); }
+    template <class T>
+    T* take(size_t count) {
+        const size_t b = pad(count * sizeof(T));
+        if (!mem_.p || used_ + b > total_) return nullptr;
+        T* p = reinterpret_cast<T*>(static_cast<char*>(mem_.p) + used_);
+        used_ += b;
+        return p;
+    }
+
+private:
+    static size_t pad(size_t b) { return (std::max<size_t>(b, 8) + 255) & ~size_t(255); }
+    DevBuf mem_;
+    size_t total_ = 0, used_ = 0;
+};
 
 // Pass 1 for the rows of `a` (all queries, or a.qlist): thr[query] from L subsets per lane
-// column over every `stride`-th block of the base.
-hipError_t thresholds(GemmArgs a, int L, uint32_t stride, int32_t* thr, hipStream_t s) {
+// column over every `stride`-th block of the base; accmax holds a.nsel * 16 * L ints.
+hipError_t thresholds(GemmArgs a, int L, uint32_t stride, int32_t* accmax, int32_t* thr,
+                      hipStream_t s) {
     const uint32_t sampleBlocks = std::max(1u, a.nBlocks / stride);
     const uint32_t subsets = 16u * (uint32_t)L;
-    DevBuf accmax;
-    hipError_t e = hipMalloc(&accmax.p, (size_t)a.nsel * subsets * 4);
-    if (e == hipSuccess)
-        e = hipMemsetD32Async((hipDeviceptr_t)accmax.p, (int)0x80000000, (size_t)a.nsel * subsets, s);
+    hipError_t e = hipMemsetD32Async((hipDeviceptr_t)accmax, (int)0x80000000, (size_t)a.nsel * subsets, s);
     if (e != hipSuccess) return e;
     a.scanBlocks = sampleBlocks;
     a.stride = stride;
@@ -711,7 +726,7 @@ hipError_t thresholds(GemmArgs a, int L, uint32_t stride, int32_t* thr, hipStrea
                        : L == 4 ? residentBlocks(bfThresholdKernel<4>) : residentBlocks(bfThresholdKernel<8>);
     a.slices = pickSlices(a.qTiles, sampleBlocks, res, 64, 512);
     a.jobs = a.qTiles * a.slices;
-    a.accmax = static_cast<int32_t*>(accmax.p);
+    a.accmax = accmax;
     switch (L) {
         case 1: e = launchThreshold<1>(a, s); break;
         case 2: e = launchThreshold<2>(a, s); break;
@@ -720,36 +735,32 @@ hipError_t thresholds(GemmArgs a, int L, uint32_t stride, int32_t* thr, hipStrea
     }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(bfReduceThresholdKernel, dim3((a.nsel + 255) / 256), dim3(256), 0, s,
-                       static_cast<const int32_t*>(accmax.p), (int)subsets, a.qlist, a.nsel, thr);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    return hipStreamSynchronize(s);  // accmax is freed on return
+                       static_cast<const int32_t*>(accmax), (int)subsets, a.qlist, a.nsel, thr);
+    return hipGetLastError();
 }
 
-// Passes 2 and 3 for the rows of `a`; retry[i] is set for the rows the select pass could
-// not finish.  With offs, list (i, s) holds exactly offs[i S + s + 1] - offs[i S + s] keys.
-hipError_t filterSelect(GemmArgs a, const uint64_t* offs, uint64_t totalSlots, uint32_t k,
-                        uint32_t* outIds, int32_t* outDists, std::vector<uint32_t>& counts,
-                        std::vector<uint32_t>& retry, hipStream_t s) {
-    DevBuf cand, cnt, flags;
-    hipError_t e = hipMalloc(&cand.p, std::max<uint64_t>(1, totalSlots) * 8);
-    if (e == hipSuccess) e = hipMalloc(&cnt.p, (size_t)a.nsel * a.slices * 4);
-    if (e == hipSuccess) e = hipMalloc(&flags.p, (size_t)a.nsel * 4);
-    if (e != hipSuccess) return e;
-    a.cand = static_cast<uint64_t*>(cand.p);
-    a.counts = static_cast<uint32_t*>(cnt.p);
+// Passes 2 and 3 for the rows of `a`, with candidate slots `cand`, list counters `cnt`
+// (a.nsel * a.slices) and flags (a.nsel); retry[i] is set for the rows the select pass
+// could not finish.  With offs, list (i, s) holds exactly offs[i S + s + 1] - offs[i S + s]
+// keys.
+hipError_t filterSelect(GemmArgs a, const uint64_t* offs, uint64_t* cand, uint32_t* cnt,
+                        uint32_t* flags, uint32_t k, uint32_t* outIds, int32_t* outDists,
+                        std::vector<uint32_t>& counts, std::vector<uint32_t>& retry, hipStream_t s) {
+    a.cand = cand;
+    a.counts = cnt;
     a.offs = offs;
     a.jobs = a.qTiles * a.slices;
     hipLaunchKernelGGL(bfFilterKernel, dim3((a.jobs + 7) / 8 * 8), dim3(256), 0, s, a);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(bfSelectKernel, dim3(a.nsel), dim3(kSelectThreads), 0, s,
-                       static_cast<const uint64_t*>(cand.p), static_cast<const uint32_t*>(cnt.p),
-                       offs, a.qlist, a.slices, a.capS, k, outIds, outDists,
-                       static_cast<uint32_t*>(flags.p));
+                       static_cast<const uint64_t*>(cand), static_cast<const uint32_t*>(cnt),
+                       offs, a.qlist, a.slices, a.capS, k, outIds, outDists, flags);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     counts.resize((size_t)a.nsel * a.slices);
     retry.resize(a.nsel);
-    e = hipMemcpyAsync(counts.data(), cnt.p, counts.size() * 4, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(retry.data(), flags.p, retry.size() * 4, hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(counts.data(), cnt, counts.size() * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(retry.data(), flags, retry.size() * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e;
 }
@@ -763,19 +774,24 @@ hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_
     if (nq == 0) return hipSuccess;
     if (k == 0 || k > (uint32_t)kTopCap) return hipErrorInvalidValue;
     std::vector<uint32_t> exact;  // queries for the one-pass kernel
+    uint32_t* exactDev = nullptr;  // room for nq ids
     auto runExact = [&]() -> hipError_t {
         if (exact.empty()) return hipSuccess;
-        DevBuf d;
-        hipError_t e = upload(d, exact.data(), exact.size() * 4, s);
+        hipError_t e = hipMemcpyAsync(exactDev, exact.data(), exact.size() * 4, hipMemcpyHostToDevice, s);
         if (e != hipSuccess) return e;
         const uint32_t ns = (uint32_t)exact.size();
         hipLaunchKernelGGL(bfKnnExactKernel, dim3((ns + kQ - 1) / kQ), dim3(64), 0, s, base, bnorm,
-                           n, queries, qnorm, static_cast<const uint32_t*>(d.p), ns, k, normsInRow,
-                           outIds, outDists);
+                           n, queries, qnorm, static_cast<const uint32_t*>(exactDev), ns, k,
+                           normsInRow, outIds, outDists);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         return hipStreamSynchronize(s);
     };
     if (n == 0) {
+        Arena ar;
+        ar.add((size_t)nq * 4);
+        hipError_t e = ar.allocate();
+        if (e != hipSuccess) return e;
+        exactDev = ar.take<uint32_t>(nq);
         for (uint32_t q = 0; q < nq; q++) exact.push_back(q);
         return runExact();
     }
@@ -800,18 +816,43 @@ hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_
     const int Lk = k <= 16 ? 1 : k <= 32 ? 2 : k <= 64 ? 4 : 8;
     const bool small = n <= kSmallBase;
     const uint32_t sampleStride = small ? 1 : 4;
-    DevBuf thr;
-    hipError_t e = hipMalloc(&thr.p, (size_t)nq * 4);
-    if (e == hipSuccess) e = thresholds(a, small ? Lk : 1, sampleStride, static_cast<int32_t*>(thr.p), s);
+    const uint32_t resFilter = residentBlocks(bfFilterKernel);
+    const uint32_t mainSlices = pickSlices(a.qTiles, a.nBlocks, resFilter, 32, 512);
+    const uint32_t mainCap = std::max(16u, perQuery / mainSlices);
+    const uint64_t mainSlots = (uint64_t)nq * mainSlices * mainCap;
+
+    // every piece sized for all nq queries: thresholds, the maxima of up to 16 * 8 subsets,
+    // candidate slots, list counters and exact offsets for up to kMaxSlices slices, flags
+    // and three query lists (looser threshold, refiltered, one-pass)
+    Arena ar;
+    ar.add((size_t)nq * 4);
+    ar.add((size_t)nq * 128 * 4);
+    ar.add(mainSlots * 8);
+    ar.add((size_t)nq * kMaxSlices * 4);
+    ar.add(((size_t)nq * kMaxSlices + 1) * 8);
+    for (int i = 0; i < 4; i++) ar.add((size_t)nq * 4);
+    hipError_t e = ar.allocate();
     if (e != hipSuccess) return e;
-    a.thr = static_cast<const int32_t*>(thr.p);
+    int32_t* thr = ar.take<int32_t>(nq);
+    int32_t* accmax = ar.take<int32_t>((size_t)nq * 128);
+    uint64_t* cand = ar.take<uint64_t>(mainSlots);
+    uint32_t* cnt = ar.take<uint32_t>((size_t)nq * kMaxSlices);
+    uint64_t* offsDev = ar.take<uint64_t>((size_t)nq * kMaxSlices + 1);
+    uint32_t* flags = ar.take<uint32_t>(nq);
+    uint32_t* looseDev = ar.take<uint32_t>(nq);
+    uint32_t* againDev = ar.take<uint32_t>(nq);
+    exactDev = ar.take<uint32_t>(nq);
+    if (!exactDev) return hipErrorOutOfMemory;
+
+    e = thresholds(a, small ? Lk : 1, sampleStride, accmax, thr, s);
+    if (e != hipSuccess) return e;
+    a.thr = thr;
     a.scanBlocks = a.nBlocks;
     a.stride = 1;
-    const uint32_t resFilter = residentBlocks(bfFilterKernel);
-    a.slices = pickSlices(a.qTiles, a.nBlocks, resFilter, 32, 512);
-    a.capS = std::max(16u, perQuery / a.slices);
+    a.slices = mainSlices;
+    a.capS = mainCap;
     std::vector<uint32_t> counts, retry;
-    e = filterSelect(a, nullptr, (uint64_t)nq * a.slices * a.capS, k, outIds, outDists, counts, retry, s);
+    e = filterSelect(a, nullptr, cand, cnt, flags, k, outIds, outDists, counts, retry, s);
     if (e != hipSuccess) return e;
 
     // Queries the select pass could not finish.  Those with fewer than k candidates get a
@@ -828,22 +869,20 @@ hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_
         if (tot < k) loose.push_back(q);
     }
     if (!loose.empty()) {
-        DevBuf list;
-        e = upload(list, loose.data(), loose.size() * 4, s);
+        e = hipMemcpyAsync(looseDev, loose.data(), loose.size() * 4, hipMemcpyHostToDevice, s);
         if (e != hipSuccess) return e;
         GemmArgs r = a;
-        r.qlist = static_cast<const uint32_t*>(list.p);
+        r.qlist = looseDev;
         r.nsel = (uint32_t)loose.size();
         r.qTiles = (r.nsel + kQueriesPerBlock - 1) / kQueriesPerBlock;
-        e = thresholds(r, 8, sampleStride, static_cast<int32_t*>(thr.p), s);
+        e = thresholds(r, 8, sampleStride, accmax, thr, s);
         if (e != hipSuccess) return e;
     }
     if (!again.empty()) {
-        DevBuf list;
-        e = upload(list, again.data(), again.size() * 4, s);
+        e = hipMemcpyAsync(againDev, again.data(), again.size() * 4, hipMemcpyHostToDevice, s);
         if (e != hipSuccess) return e;
         GemmArgs r = a;
-        r.qlist = static_cast<const uint32_t*>(list.p);
+        r.qlist = againDev;
         r.nsel = (uint32_t)again.size();
         r.qTiles = (r.nsel + kQueriesPerBlock - 1) / kQueriesPerBlock;
         r.scanBlocks = r.nBlocks;
@@ -851,10 +890,9 @@ hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_
         r.slices = pickSlices(r.qTiles, r.nBlocks, resFilter, kMaxSlices, 64);
         r.capS = 0;  // count only
         std::vector<uint32_t> c1, r1;
-        e = filterSelect(r, nullptr, 1, k, outIds, outDists, c1, r1, s);
+        e = filterSelect(r, nullptr, cand, cnt, flags, k, outIds, outDists, c1, r1, s);
         if (e != hipSuccess) return e;
         std::vector<uint64_t> offs(1, 0);
-        std::vector<uint32_t> keepRows;
         for (uint32_t x = 0; x < r.nsel; x++) {
             uint64_t tot = 0;
             for (uint32_t j = 0; j < r.slices; j++) tot += c1[(size_t)x * r.slices + j] & 0x7FFFFFFFu;
@@ -863,11 +901,17 @@ hipError_t launchBruteForceKnn(const int8_t* base, const int32_t* bnorm, uint32_
             for (uint32_t j = 0; j < r.slices; j++)
                 offs.push_back(offs.back() + (ok ? (c1[(size_t)x * r.slices + j] & 0x7FFFFFFFu) : 0));
         }
-        DevBuf doffs;
-        e = upload(doffs, offs.data(), offs.size() * 8, s);
+        e = hipMemcpyAsync(offsDev, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, s);
         if (e != hipSuccess) return e;
+        // the main pass's lists are done with; more keys than they held get their own slots
+        DevBuf more;
+        uint64_t* cand2 = cand;
+        if (offs.back() > mainSlots) {
+            if ((e = hipMalloc(&more.p, offs.back() * 8)) != hipSuccess) return e;
+            cand2 = static_cast<uint64_t*>(more.p);
+        }
         std::vector<uint32_t> c2, r2;
-        e = filterSelect(r, static_cast<const uint64_t*>(doffs.p), offs.back(), k, outIds, outDists, c2, r2, s);
+        e = filterSelect(r, offsDev, cand2, cnt, flags, k, outIds, outDists, c2, r2, s);
         if (e != hipSuccess) return e;
         for (uint32_t x = 0; x < r.nsel; x++) {
             const bool wasOk = offs[(size_t)(x + 1) * r.slices] > offs[(size_t)x * r.slices];
