@@ -208,8 +208,9 @@ typedef struct pann_batch_config {
     int32_t max_k;           /* output stride */
     int32_t waves_per_block; /* 1, 2 or 4 (0: the most that fit 64 KB of LDS) */
     int32_t lds_filter_bits; /* seen filters up to 2^bits slots live in LDS, 2..12 (0: 10) */
-    uint32_t visited_slots;  /* visited / is_inw hash slots per query (LDS, or HBM from 2048 /
-                                4096 slots), power of two >= 64 (0: derived) */
+    uint32_t visited_slots;  /* flat graphs: visited hash slots per query (LDS, or HBM from
+                                2048 slots), power of two >= 64 (0: derived); HNSW searches
+                                keep no such table and ignore it (reads back 0) */
     uint32_t prefetch;       /* next-row prefetch depth, 0, 1 or 2: at the start of a hop, load
                                 the rows of the first (and second) unvisited frontier entries
                                 after the expanded node, the likely next nodes (flat graphs,

@@ -72,106 +72,16 @@ struct WaveList {
     }
 };
 
-__device__ __forceinline__ uint32_t inwSlot(uint32_t id, uint32_t logCap) {
-    return (id * 2654435761u) >> (32 - logCap);
-}
-
-// is_inw, open addressing.  In LDS (GROUPS false) an id takes the first free slot from its
-// hash slot on, one 4-B probe at a time.  In HBM (GROUPS true) it hashes to an aligned
-// group of four slots and takes the first free slot from there on, group by group, so one
-// 16-B load answers most lookups.  The layer search stops before the table fills, so every
-// walk meets a free slot.
-template <bool GROUPS>
-__device__ __forceinline__ bool inwContains(const uint32_t* inw, uint32_t id, uint32_t logCap,
-                                            uint32_t mask) {
-    if constexpr (!GROUPS) {
-        uint32_t h = inwSlot(id, logCap);
-        while (true) {
-            const uint32_t x = inw[h];
-            if (x == id) return true;
-            if (x == kEmpty) return false;
-            h = (h + 1) & mask;
-        }
-    }
-    uint32_t g = inwSlot(id, logCap) & ~3u;
-    while (true) {
-        const uint4 k4 = *reinterpret_cast<const uint4*>(inw + g);
-        const uint32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (kk[j] == id) return true;
-            if (kk[j] == kEmpty) return false;
-        }
-        g = (g + 4) & mask;
-    }
-}
-
-// Adds id (one lane, nothing else writing): true if it was new.
-template <bool GROUPS>
-__device__ __forceinline__ bool inwInsertOne(uint32_t* inw, uint32_t id, uint32_t logCap, uint32_t mask) {
-    if constexpr (!GROUPS) {
-        uint32_t h = inwSlot(id, logCap);
-        while (inw[h] != kEmpty && inw[h] != id) h = (h + 1) & mask;
-        if (inw[h] == id) return false;
-        inw[h] = id;
-        return true;
-    }
-    uint32_t g = inwSlot(id, logCap) & ~3u;
-    while (true) {
-        const uint4 k4 = *reinterpret_cast<const uint4*>(inw + g);
-        const uint32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (kk[j] == id) return false;
-            if (kk[j] == kEmpty) {
-                inw[g + j] = id;
-                return true;
-            }
-        }
-        g = (g + 4) & mask;
-    }
-}
-
-// Adds id, not yet present, while other lanes add other ids: a slot is claimed by
-// compare-and-swap, and a lost race moves on to the next free slot.
-template <bool GROUPS>
-__device__ __forceinline__ void inwInsertShared(uint32_t* inw, uint32_t id, uint32_t logCap, uint32_t mask) {
-    if constexpr (!GROUPS) {
-        uint32_t h = inwSlot(id, logCap);
-        while (true) {
-            const uint32_t old = atomicCAS(&inw[h], kEmpty, id);
-            if (old == kEmpty || old == id) return;
-            h = (h + 1) & mask;
-        }
-    }
-    uint32_t g = inwSlot(id, logCap) & ~3u;
-    while (true) {
-        const uint4 k4 = *reinterpret_cast<const uint4*>(inw + g);
-        const uint32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
-        int j = 0;
-        while (j < 4 && kk[j] != kEmpty) j++;
-        if (j == 4) {
-            g = (g + 4) & mask;
-            continue;
-        }
-        if (atomicCAS(&inw[g + j], kEmpty, id) == kEmpty) return;
-        // another lane took that slot: read the group again
-    }
-}
-
 struct LayerState {
     int comps, edges, hops, flags;
-    uint32_t inwCount;
 };
 
 // Expands node u on `layer`: row gather, seen filter, distances, in-order acceptance.
-// IG: is_inw in HBM, so membership is read while the vectors are in flight (in LDS it is
-// read after the distances, which keeps fewer registers live across the loads).
-template <int WR, int MAXCH, bool G, bool IG>
+template <int WR, int MAXCH, bool G>
 __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* filt, uint32_t fmask,
                            uint32_t u, int layer, int ef, const int4& qv, int qq,
-                           WaveList<WR>& W, WaveList<WR>& cand, LayerState& st, uint32_t* inw,
-                           uint32_t capInw, uint32_t logCap, int lane, uint64_t lanesBelow) {
+                           WaveList<WR>& W, WaveList<WR>& cand, LayerState& st, int lane,
+                           uint64_t lanesBelow) {
     const int grp = lane >> 3, sub = lane & 7;
     uint32_t* pend = reinterpret_cast<uint32_t*>(lds + L.lds.pend);
     unsigned char* tag = lds + L.lds.tag;
@@ -202,12 +112,10 @@ __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* fi
     // distances, then move candidate j to lane j of block j/64
     int dist[MAXCH];
     uint32_t id[MAXCH];
-    bool valid[MAXCH], memb[MAXCH], acc[MAXCH];
+    bool valid[MAXCH];
 #pragma unroll
     for (int b = 0; b < MAXCH; b++) {
         valid[b] = false;
-        memb[b] = false;
-        acc[b] = false;
         dist[b] = 0;
         id[b] = kEmpty;
         if ((uint32_t)b * 64 >= nP) continue;
@@ -220,11 +128,6 @@ __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* fi
             const uint32_t i = (uint32_t)r * 8 + grp;
             if ((uint32_t)r < rounds && i < nb)
                 v[r] = *reinterpret_cast<const int4*>(L.vec + (size_t)pend[b * 64 + i] * kVecBytes + sub * 16);
-        }
-        // membership before this row's acceptances
-        if (IG && (uint32_t)lane < nb) {
-            id[b] = pend[b * 64 + lane];
-            memb[b] = inwContains<IG>(inw, id[b], logCap, capInw - 1);
         }
         int part8[8];
         const int normMask = (L.normsInRow && sub == 7) ? -1 : 0;
@@ -239,14 +142,17 @@ __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* fi
         if ((uint32_t)lane < nb) {
             valid[b] = true;
             dist[b] = qq + mine;
-            if (!IG) {
-                id[b] = pend[b * 64 + lane];
-                memb[b] = inwContains<IG>(inw, id[b], logCap, capInw - 1);
-            }
+            id[b] = pend[b * 64 + lane];
         }
     }
 
-    // in-order acceptance; only neighbours that can still pass are visited
+    // In-order acceptance; only neighbours that can still pass are visited.  is_inw (the
+    // ids that ever entered W on this layer) is not kept: a neighbour that passes the
+    // threshold is in is_inw exactly when it is in W now.  W only grows until it holds ef
+    // entries and then keeps ef, each insertion dropping the farthest, so W's farthest
+    // distance never rises after the first drop; an id dropped from W lay at or beyond it,
+    // cannot pass `d < farthest` again, and every other id of is_inw is still in W, where
+    // its key (d, id) is found at its insertion position.
     int pos = -1;
     while (true) {
         const bool full = W.size >= ef;
@@ -256,10 +162,11 @@ __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* fi
         for (int b = 0; b < MAXCH; b++) {
             if (j >= 0) break;
             const int e = b * 64 + lane;
-            const uint64_t m = ballot(valid[b] && e > pos && !memb[b] && (!full || (uint32_t)dist[b] < wl));
+            const uint64_t m = ballot(valid[b] && e > pos && (!full || (uint32_t)dist[b] < wl));
             if (m) j = b * 64 + __ffsll((unsigned long long)m) - 1;
         }
         if (j < 0) break;
+        pos = j;
         int dj = 0;
         uint32_t idj = 0;
 #pragma unroll
@@ -269,39 +176,24 @@ __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* fi
                 idj = (uint32_t)readlane((int)id[b], j & 63);
             }
         const uint64_t key = ((uint64_t)(uint32_t)dj << 32) | idj;
-        W.insertAt(W.countLess(key, lane), key, lane);
+        const int at = W.countLess(key, lane);
+        if (at < W.size && W.get(at) == key) continue;  // in W: already in is_inw
+        W.insertAt(at, key, lane);
         if (!cand.hasDist((uint32_t)dj, lane))
             cand.insertAt(cand.countLess((uint64_t)(uint32_t)dj << 32, lane), key, lane);
         if (W.size > ef) W.size--;
         if (cand.size > ef) cand.size--;
-#pragma unroll
-        for (int b = 0; b < MAXCH; b++) {
-            memb[b] = memb[b] || (valid[b] && id[b] == idj);
-            acc[b] = acc[b] || (b * 64 + lane == j);
-        }
-        pos = j;
     }
-
-    // record the accepted ids in is_inw
-    uint32_t added = 0;
-#pragma unroll
-    for (int b = 0; b < MAXCH; b++) {
-        added += (uint32_t)__popcll(ballot(acc[b]));
-        if (acc[b]) inwInsertShared<IG>(inw, id[b], logCap, capInw - 1);
-    }
-    st.inwCount += added;
     __builtin_amdgcn_wave_barrier();
 }
 
-// IG: layer 0's is_inw in this wave's HBM region (ginw) rather than in LDS; the upper
-// layers (ef = 1) then keep theirs in LDS, kInwUpperCap slots.
-template <int WR, int MAXCH, bool FGLOBAL, bool IG>
-__device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt, uint32_t* ginw,
-                        uint32_t q, int Ls, int k, int64_t visit) {
+// FGLOBAL: layer 0's seen filter in this wave's HBM region (gfilt) rather than in LDS.
+template <int WR, int MAXCH, bool FGLOBAL>
+__device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt, uint32_t q,
+                        int Ls, int k, int64_t visit) {
     const int lane = laneId();
     const int sub = lane & 7;
     const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint32_t* ldsInw = reinterpret_cast<uint32_t*>(lds + L.lds.inw);
     uint32_t* ldsFilt = reinterpret_cast<uint32_t*>(lds + L.lds.filt);
     // rows may hold |p|^2 in bytes 124..127 (lane 7's .w): masked out of the products
     const int normMask = (L.normsInRow && sub == 7) ? -1 : 0;
@@ -310,14 +202,12 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
     const int qq = groupSum8(sliceNorm(qv));
     const int64_t lim0 = (visit > 0 && visit < (int64_t)L.n) ? visit : (int64_t)L.n;
     const int bits0 = hnswFilterBits(Ls);
-    uint32_t capInw0 = hnswInwCap(Ls);
-    if (capInw0 > L.capInw) capInw0 = L.capInw;
 
     uint32_t eps[8];
     int numEps = L.numEntries;
 #pragma unroll
     for (int e = 0; e < 8; e++) eps[e] = L.entries[e];
-    LayerState st{0, 0, 0, 0, 0};
+    LayerState st{0, 0, 0, 0};
     WaveList<WR> W, cand;
     W.clear();
     cand.clear();
@@ -331,16 +221,6 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
         uint32_t* filt = globalFilter ? gfilt : ldsFilt;
         if (globalFilter) filtClear<true>(filt, bits, lane);
         else filtClear<false>(filt, bits, lane);
-        const bool hbmInw = IG && layer == 0;
-        uint32_t* inw = hbmInw ? ginw : ldsInw;
-        const uint32_t capInw = (IG && layer > 0) ? min(capInw0, kInwUpperCap) : capInw0;
-        const uint32_t logCap = (uint32_t)ceilLog2(capInw);
-        {
-            const uint4 e4 = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
-            uint4* i4 = reinterpret_cast<uint4*>(inw);
-            for (uint32_t i = lane; i < capInw / 4; i += 64) i4[i] = e4;
-        }
-        st.inwCount = 0;
         W.clear();
         cand.clear();
         __builtin_amdgcn_wave_barrier();
@@ -351,9 +231,6 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
                 const uint32_t slot = (uint32_t)hash64(pe) & fmask;
                 if (globalFilter) filtStore<true>(filt, slot, pe);
                 else filtStore<false>(filt, slot, pe);
-                const bool added = hbmInw ? inwInsertOne<true>(inw, pe, logCap, capInw - 1)
-                                          : inwInsertOne<false>(inw, pe, logCap, capInw - 1);
-                if (added) st.inwCount++;
             }
             if (globalFilter) __builtin_amdgcn_s_waitcnt(0);
             __builtin_amdgcn_wave_barrier();
@@ -364,7 +241,6 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
             W.insertAt(W.countLess(key, lane), key, lane);
             st.comps++;
         }
-        st.inwCount = (uint32_t)readlane((int)st.inwCount, 0);
 
         int64_t cnt = 0;
         while (cand.size > 0) {
@@ -374,27 +250,13 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
             if (++cnt > limit) break;
             cand.popFront(lane);
             st.hops++;
-            if (globalFilter) {
-                if (hbmInw)
-                    expandNode<WR, MAXCH, true, IG>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq,
-                                                    W, cand, st, inw, capInw, logCap, lane, lanesBelow);
-                else
-                    expandNode<WR, MAXCH, true, false>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq,
-                                                       W, cand, st, inw, capInw, logCap, lane, lanesBelow);
-            } else {
-                if (hbmInw)
-                    expandNode<WR, MAXCH, false, IG>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq,
-                                                     W, cand, st, inw, capInw, logCap, lane, lanesBelow);
-                else
-                    expandNode<WR, MAXCH, false, false>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv,
-                                                        qq, W, cand, st, inw, capInw, logCap, lane, lanesBelow);
-            }
-            if (st.inwCount + 64u * MAXCH >= capInw) {
-                st.flags |= kFlagOverflow;
-                break;
-            }
+            if (globalFilter)
+                expandNode<WR, MAXCH, true>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq, W, cand,
+                                            st, lane, lanesBelow);
+            else
+                expandNode<WR, MAXCH, false>(L, lds, filt, fmask, (uint32_t)c0, layer, ef, qv, qq, W, cand,
+                                             st, lane, lanesBelow);
         }
-        if (st.flags & kFlagOverflow) break;
         if (layer > 0) {
             eps[0] = (uint32_t)W.get(W.size - 1);
             numEps = 1;
@@ -436,7 +298,7 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
 // the 1M HNSW run 1.11x / 1.17x / 1.3x / 1.2x faster than with the registers the compiler
 // would take (one-register lists held to 96 VGPRs run 1.15x slower).  Eight-register lists
 // spill too much at 168 (ef=500: 52.3 ms held, 47.7 ms not).
-template <int WR, int MAXCH, bool IG>
+template <int WR, int MAXCH>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(MAXCH == 1 && WR <= 2 ? 4 : MAXCH == 1 && WR >= 5 && WR <= 7 ? 3 : 1, 8)))
 void beamHnswKernel(HnswLaunch L) {
@@ -447,7 +309,6 @@ void beamHnswKernel(HnswLaunch L) {
     const uint32_t waveGlobal = blockIdx.x * wavesPerBlock + waveInBlock;
     unsigned char* lds = smem + (size_t)waveInBlock * L.ldsPerWave;
     uint32_t* gfilt = L.gfilt ? L.gfilt + ((size_t)waveGlobal << L.gbitsMax) : nullptr;
-    uint32_t* ginw = IG ? L.ginw + (size_t)waveGlobal * L.capInw : nullptr;
 
     // queue mode ends on kQueueEmpty, strided mode past the last query
     const uint32_t qEnd = L.queue ? kQueueEmpty : L.nq;
@@ -465,9 +326,9 @@ void beamHnswKernel(HnswLaunch L) {
             (fb > L.fbitsLds && (L.gfilt == nullptr || fb > L.gbitsMax))) {
             rejectQuery(L, q, lane);
         } else if (fb <= L.fbitsLds) {
-            hnswOne<WR, MAXCH, false, IG>(L, lds, nullptr, ginw, q, Ls, k, visit);
+            hnswOne<WR, MAXCH, false>(L, lds, nullptr, q, Ls, k, visit);
         } else {
-            hnswOne<WR, MAXCH, true, IG>(L, lds, gfilt, ginw, q, Ls, k, visit);
+            hnswOne<WR, MAXCH, true>(L, lds, gfilt, q, Ls, k, visit);
         }
         q = L.queue ? takeQuery(L.queue, L.order, L.nq, lane) : q + L.totalWaves;
     }
@@ -477,27 +338,13 @@ void beamHnswKernel(HnswLaunch L) {
 template <int WR, int MAXCH>
 hipError_t launchT(const HnswLaunch& L, int wpb, int blocks, hipStream_t s) {
     const size_t lds = (size_t)L.ldsPerWave * wpb;
-    if (L.ginw) {
-        if constexpr (WR >= 2) {
-            hipLaunchKernelGGL((beamHnswKernel<WR, MAXCH, true>), dim3(blocks), dim3(64 * wpb), lds, s, L);
-        } else {
-            return hipErrorInvalidValue;
-        }
-    } else {
-        hipLaunchKernelGGL((beamHnswKernel<WR, MAXCH, false>), dim3(blocks), dim3(64 * wpb), lds, s, L);
-    }
+    hipLaunchKernelGGL((beamHnswKernel<WR, MAXCH>), dim3(blocks), dim3(64 * wpb), lds, s, L);
     return hipGetLastError();
 }
 
 template <int WR, int MAXCH>
-hipError_t occT(bool inwHbm, int wpb, size_t lds, int* bpc) {
-    if (inwHbm) {
-        if constexpr (WR >= 2)
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, beamHnswKernel<WR, MAXCH, true>, 64 * wpb, lds);
-        else
-            return hipErrorInvalidValue;
-    }
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, beamHnswKernel<WR, MAXCH, false>, 64 * wpb, lds);
+hipError_t occT(int wpb, size_t lds, int* bpc) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, beamHnswKernel<WR, MAXCH>, 64 * wpb, lds);
 }
 
 }  // namespace
@@ -536,8 +383,8 @@ hipError_t launchBeamHnsw(const HnswLaunch& L, int wr, int maxch, int wavesPerBl
     PANN_HNSW_DISPATCH(launchT, L, wavesPerBlock, blocks, stream)
 }
 
-hipError_t beamHnswOccupancy(int wr, int maxch, bool inwHbm, int wavesPerBlock, size_t ldsBytes, int* bpc) {
-    PANN_HNSW_DISPATCH(occT, inwHbm, wavesPerBlock, ldsBytes, bpc)
+hipError_t beamHnswOccupancy(int wr, int maxch, int wavesPerBlock, size_t ldsBytes, int* bpc) {
+    PANN_HNSW_DISPATCH(occT, wavesPerBlock, ldsBytes, bpc)
 }
 
 }  // namespace pann

@@ -644,9 +644,7 @@ int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_searc
     const int maxch = G <= 64 ? 1 : G <= 128 ? 2 : 4;
     const int bits0 = hnswFilterBits(Ls);
     const bool ldsFilter = bits0 <= kFilterBitsLds;
-    const bool inwHbm = wr >= 2 && hnswInwCap(Ls) >= kInwHbmMinCap;
-    const uint32_t capInw = inwHbm ? hnswInwCap(Ls) : hnswInwCapFit(Ls, maxch, ldsFilter ? bits0 : 0, kMaxLdsPerBlock);
-    const HnswLds lay = hnswLdsLayout(maxch, inwHbm ? kInwUpperCap : capInw, ldsFilter ? bits0 : 0);
+    const HnswLds lay = hnswLdsLayout(maxch, ldsFilter ? bits0 : 0);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "Ls=%d needs %u bytes of LDS per query", Ls, lay.total);
 
@@ -658,7 +656,7 @@ int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_searc
     s->wr = wr;
     int bpc = 0, cus = 0;
     r = pickWavesPerBlock(lay.total, [&](int wpb, size_t lds, int* b) {
-            return beamHnswOccupancy(wr, maxch, inwHbm, wpb, lds, b); }, &s->wavesPerBlock, &bpc);
+            return beamHnswOccupancy(wr, maxch, wpb, lds, b); }, &s->wavesPerBlock, &bpc);
     if (r == PANN_OK) r = deviceCUs(&cus);
     if (r != PANN_OK) {
         delete s;
@@ -683,7 +681,6 @@ int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_searc
     H.LsMax = Ls;
     H.fbitsLds = ldsFilter ? bits0 : 2;
     H.gbitsMax = ldsFilter ? 0 : bits0;
-    H.capInw = capInw;
     H.ldsPerWave = lay.total;
     H.lds = lay;
     H.queue = nullptr;
@@ -696,14 +693,6 @@ int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_searc
         }
     }
     H.gfilt = s->gfilt;
-    if (inwHbm) {
-        he = hipMalloc(&s->gvis, (size_t)totalWaves * capInw * 4);
-        if (he != hipSuccess) {
-            pann_searcher_free(s);
-            return setError(PANN_ERR_HIP, "allocating is_inw tables: %s", hipGetErrorString(he));
-        }
-        H.ginw = s->gvis;
-    }
     r = allocOutputs(s->out, nq, (uint32_t)std::max(1, cfg->k));
     if (r != PANN_OK) {
         pann_searcher_free(s);

@@ -20,7 +20,7 @@ using namespace pann;
 // Ls-descending order, drained by a kernel sized for LsHi.
 struct BatchPart {
     int LsLo = 0, LsHi = 0;
-    int fastKind = 0;  // flat: FlatLaunch::fast (0: general); HNSW: 1 with is_inw in HBM
+    int fastKind = 0;  // flat: FlatLaunch::fast (0: general); HNSW: 0
     int wr = 1;        // HNSW: list width in wavefront registers
     int wavesPerBlock = 1, blocks = 0;
     int wpq = 1;       // flat general form: wavefronts per query (blocks hold one query)
@@ -121,10 +121,9 @@ int sizeLaunch(pann_batch* b, int fastKind, int wr, uint32_t ldsPerWave, int* wp
     int cus = 0;
     int r = deviceCUs(&cus);
     if (r != PANN_OK) return r;
-    // HNSW: fastKind != 0 selects the kernel with is_inw in HBM
     auto occ = [&](int w, size_t lds, int* bpc) {
         return b->kind == 0 ? beamFlatOccupancy(b->maxch, fastKind, w, lds, bpc)
-                            : beamHnswOccupancy(wr, b->maxch, fastKind != 0, w, lds, bpc);
+                            : beamHnswOccupancy(wr, b->maxch, w, lds, bpc);
     };
     int wpb = b->cfg.waves_per_block;
     int bpc = 0;
@@ -207,36 +206,28 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int*
 }
 
 // Sizes one HNSW launch for beams up to LsHi (as planFlatPart).  Records the launch's
-// global filter bits in *gbits and its HBM is_inw slots per wave (0: LDS) in *ginwSlots.
-int planHnswPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int* gbits, uint32_t* ginwSlots) {
+// global filter bits in *gbits.  HNSW launches keep no visited table (visited_slots is
+// not used and reads back 0).
+int planHnswPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int* gbits) {
     const pann_batch_config& c = b->cfg;
     const pann_hnsw* h = b->hnsw;
     const int wr = hnswListWidth(LsHi, h->numEntries);
     if (!wr) return setError(PANN_ERR_CAPACITY, "HNSW max_Ls=%d above the supported 503", LsHi);
     const int fbMax = hnswFilterBits(LsHi);
     const int fbLds = std::max(2, std::min(c.lds_filter_bits ? c.lds_filter_bits : kFilterBitsLds, fbMax));
-    // is_inw in HBM from kInwHbmMinCap slots on (then at the full derived capacity)
-    const uint32_t derived = hnswInwCap(LsHi);
-    const uint32_t want = !c.visited_slots ? derived : top ? c.visited_slots : std::min(c.visited_slots, derived);
-    const bool inwHbm = wr >= 2 && want >= kInwHbmMinCap;
-    const uint32_t capInw = c.visited_slots || inwHbm ? want : hnswInwCapFit(LsHi, b->maxch, fbLds, kMaxLdsPerBlock);
-    if (capInw < 256u * b->maxch)
-        return setError(PANN_ERR_CAPACITY, "visited_slots %u too small for rows of %u", capInw,
-                        std::max(h->G0, h->GA));
     // a launch whose shortest beam already needs a larger layer-0 filter keeps only the
     // 4-slot upper-layer filter in LDS
     const int fbLaunch = hnswFilterBits(LsLo + 1) > fbLds ? 2 : fbLds;
-    const HnswLds lay = hnswLdsLayout(b->maxch, inwHbm ? kInwUpperCap : capInw, fbLaunch);
+    const HnswLds lay = hnswLdsLayout(b->maxch, fbLaunch);
     if (lay.total > kMaxLdsPerBlock)
         return setError(PANN_ERR_CAPACITY, "max_Ls=%d needs %u bytes of LDS per query", LsHi, lay.total);
     P->LsLo = LsLo;
     P->LsHi = LsHi;
     P->wr = wr;
-    P->fastKind = inwHbm ? 1 : 0;
+    P->fastKind = 0;
     int r = sizeLaunch(b, P->fastKind, wr, lay.total, &P->wavesPerBlock, &P->blocks);
     if (r != PANN_OK) return r;
     *gbits = fbMax > fbLds ? fbMax : 0;
-    *ginwSlots = inwHbm ? capInw : 0;
     HnswLaunch& H = P->H;
     H.vec = h->nodeVec;
     H.normsInRow = h->dim <= (uint32_t)kNormInRowMaxDim;
@@ -253,12 +244,11 @@ int planHnswPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int*
     H.LsMax = LsHi;
     H.fbitsLds = fbLaunch;
     H.gbitsMax = *gbits;
-    H.capInw = capInw;
     H.ldsPerWave = lay.total;
     H.lds = lay;
     if (top) {
         b->cfg.lds_filter_bits = fbLds;
-        b->cfg.visited_slots = capInw;
+        b->cfg.visited_slots = 0;
     }
     return PANN_OK;
 }
@@ -386,11 +376,10 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
         if (hi < LsMax) his.push_back(hi);
     his.push_back(LsMax);
     b->parts.resize(his.size());
-    size_t gfiltBytes = 0, gvisBytes = 0;
+    size_t gfiltBytes = 0;
     for (size_t i = 0; i < his.size(); i++) {
         int gbits = 0;
-        uint32_t inwSlots = 0;
-        r = planHnswPart(b, i ? his[i - 1] : 0, his[i], i + 1 == his.size(), &b->parts[i], &gbits, &inwSlots);
+        r = planHnswPart(b, i ? his[i - 1] : 0, his[i], i + 1 == his.size(), &b->parts[i], &gbits);
         if (r != PANN_OK) {
             pann_batch_free(b);
             return r;
@@ -399,11 +388,9 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
         const uint32_t waves = (uint32_t)P.blocks * P.wavesPerBlock;
         b->totalWaves = std::max(b->totalWaves, waves);
         P.gfiltOff = gfiltBytes;
-        P.gvisOff = gvisBytes;
         if (gbits) gfiltBytes += ((size_t)waves << gbits) * 4;
-        gvisBytes += (size_t)waves * inwSlots * 4;
     }
-    r = finishParts(b, gfiltBytes, gvisBytes);
+    r = finishParts(b, gfiltBytes, 0);
     if (r != PANN_OK) {
         pann_batch_free(b);
         return r;
@@ -414,7 +401,6 @@ int pann_batch_prepare_hnsw(const pann_hnsw* h, uint32_t max_queries, const pann
         H.queries = b->queries;
         H.qp = b->params;
         H.gfilt = H.gbitsMax ? reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(b->gfilt) + P.gfiltOff) : nullptr;
-        H.ginw = P.fastKind ? reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(b->gvis) + P.gvisOff) : nullptr;
         H.outIds = b->out.ids;
         H.outDists = b->out.dists;
         H.outStride = b->out.stride;
@@ -597,15 +583,17 @@ int pann_batch_autotune(const pann_vectors* base, const pann_graph* g, const pan
                         int32_t max_Ls, int32_t max_k, pann_batch_config* best, float* best_ms) {
     if (!best || (!g && !h) || (g && !base) || !host_rows || !params || nq == 0)
         return setError(PANN_ERR_ARG, "invalid autotune arguments");
-    const uint32_t derived = h ? hnswInwCap(max_Ls) : flatVisitedCap(max_Ls);
+    // visited-table slots: HNSW searches keep none (one value, 0)
+    const uint32_t derived = h ? 0 : flatVisitedCap(max_Ls);
+    const std::vector<uint32_t> slotChoices =
+        h ? std::vector<uint32_t>{0} : std::vector<uint32_t>{derived, derived / 2, derived * 2};
     const int fbMax = h ? hnswFilterBits(max_Ls) : flatFilterBits(max_Ls);
     std::vector<pann_batch_config> cands;
     for (int wpb : {1, 2, 4})
         for (int fb : {10, 11, 12}) {
             if (fb > std::max(fbMax, 10)) continue;
-            for (uint32_t vs : {derived, derived / 2, derived * 2}) {
-                if (vs < (h ? 512u : 256u) || vs > 16384) continue;
-                if (!h && vs < 2u * (uint32_t)max_Ls + 64) continue;
+            for (uint32_t vs : slotChoices) {
+                if (!h && (vs < 256u || vs > 16384 || vs < 2u * (uint32_t)max_Ls + 64)) continue;
                 for (uint32_t pf : {0u, 1u, 2u}) {
                     if (pf && (h || max_Ls > kFastLsMax)) continue;
                     // several wavefronts per query: flat graphs whose top class is the

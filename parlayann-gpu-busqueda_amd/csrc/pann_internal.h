@@ -191,8 +191,6 @@ struct FlatLaunch {
 // ---- HNSW ----
 // filter bits of HNSW::search at ef: ef > 2 ? 2*ceil(log2 ef) - 2 : 2   cuda/busqueda_hnsw.cu:556-557
 __host__ __device__ inline int hnswFilterBits(int ef) { return ef > 2 ? 2 * ceilLog2(ef) - 2 : 2; }
-// capacity of the exact "entered the workset" set of one layer search
-__host__ __device__ inline uint32_t hnswInwCap(int Ls) { return 1u << ceilLog2((uint64_t)16 * Ls + 256); }
 // Width in wavefront registers (64 entries each) of the HNSW kernel's W and cand lists for
 // beams up to Ls: Ls + 9 entries (the entry points and a margin) must fit; 0 = too wide.
 inline int hnswListWidth(int Ls, uint32_t numEntries) {
@@ -201,37 +199,24 @@ inline int hnswListWidth(int Ls, uint32_t numEntries) {
          : need <= 384 ? 6 : need <= 448 ? 7 : need <= 512 ? 8 : 0;
 }
 
-// Launches whose is_inw set has at least this many slots (ef > 112) keep layer 0's in HBM
-// (HnswLaunch::ginw): in LDS it would leave one wavefront per SIMD at ef around 256.  Their
-// upper layers (ef = 1) keep a kInwUpperCap-slot set in LDS, the layout's inw region.
-constexpr uint32_t kInwHbmMinCap = 4096;
-constexpr uint32_t kInwUpperCap = 1024;
-
+// The HNSW kernel keeps no is_inw table (cuda/busqueda_hnsw.cu:39-60): a neighbour that
+// passes the workset threshold entered the workset before exactly when it is in the
+// workset now, which the kernel reads off its register-resident list.
 struct HnswLds {
     uint32_t pend;   // pending ids (64*maxch u32)
-    uint32_t inw;    // is_inw hash (capInw u32)
     uint32_t filt;   // seen filter when in LDS (max(4, 1 << fbitsLds) u32)
     uint32_t tag;    // kTagSlots u8
     uint32_t total;
 };
 
-__host__ __device__ inline HnswLds hnswLdsLayout(int maxch, uint32_t capInw, int fbitsLds) {
+__host__ __device__ inline HnswLds hnswLdsLayout(int maxch, int fbitsLds) {
     HnswLds L;
     uint32_t o = 0;
     L.pend = o; o += 4u * 64u * (uint32_t)maxch;
-    L.inw = o;  o += 4u * capInw;
     L.filt = o; o += 4u * (fbitsLds > 2 ? (1u << fbitsLds) : 4u);
     L.tag = o;  o += kTagSlots;
     L.total = alignUp(o, 16);
     return L;
-}
-
-// Default is_inw capacity: hnswInwCap(Ls), halved until the per-wavefront layout fits
-// ldsLimit bytes (ef around 500); a query that would fill the table is flagged OVERFLOW.
-__host__ __device__ inline uint32_t hnswInwCapFit(int Ls, int maxch, int fbitsLds, uint32_t ldsLimit) {
-    uint32_t cap = hnswInwCap(Ls);
-    while (cap > 256u * (uint32_t)maxch && hnswLdsLayout(maxch, cap, fbitsLds).total > ldsLimit) cap >>= 1;
-    return cap;
 }
 
 struct HnswLaunch {
@@ -250,10 +235,9 @@ struct HnswLaunch {
     int32_t Ls, k;
     int64_t visit;             // <= 0: none
     int32_t LsMax, fbitsLds, gbitsMax;
-    uint32_t capInw, ldsPerWave;
+    uint32_t ldsPerWave;
     HnswLds lds;
     uint32_t* gfilt;
-    uint32_t* ginw;            // non-null: capInw is_inw slots per wave in HBM
     uint32_t* queue;
     const uint32_t* order;
     uint32_t totalWaves;
@@ -268,8 +252,7 @@ struct HnswLaunch {
 
 hipError_t launchBeamHnsw(const HnswLaunch& L, int wr, int maxch, int wavesPerBlock, int blocks,
                           hipStream_t stream);
-hipError_t beamHnswOccupancy(int wr, int maxch, bool inwHbm, int wavesPerBlock, size_t ldsBytes,
-                             int* blocksPerCU);
+hipError_t beamHnswOccupancy(int wr, int maxch, int wavesPerBlock, size_t ldsBytes, int* blocksPerCU);
 
 // Launchers implemented in the .hip translation units.
 hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int blocks,
