@@ -51,11 +51,12 @@ struct WaveList {
         for (int r = 0; r < WR; r++) m |= ballot(r * 64 + lane < size && (uint32_t)(v[r] >> 32) == d);
         return m != 0;
     }
+    // Lane shifts are whole-wave DPP moves (no LDS round trip).
     __device__ void insertAt(int pos, uint64_t key, int lane) {
 #pragma unroll
         for (int r = WR - 1; r >= 0; r--) {
-            uint64_t prev = shflUp64(v[r], 1);
-            if (lane == 0) prev = (r > 0) ? readlane64(v[r > 0 ? r - 1 : 0], 63) : ~0ull;
+            const uint64_t first = r > 0 ? readlane64(v[r > 0 ? r - 1 : 0], 63) : ~0ull;
+            const uint64_t prev = prevLane64(v[r], first, lane);
             const int e = r * 64 + lane;
             v[r] = e > pos ? prev : (e == pos ? key : v[r]);
         }
@@ -64,9 +65,8 @@ struct WaveList {
     __device__ void popFront(int lane) {
 #pragma unroll
         for (int r = 0; r < WR; r++) {
-            uint64_t next = shflDown64(v[r], 1);
-            if (lane == 63) next = (r + 1 < WR) ? readlane64(v[r + 1 < WR ? r + 1 : r], 0) : ~0ull;
-            v[r] = next;
+            const uint64_t last = r + 1 < WR ? readlane64(v[r + 1 < WR ? r + 1 : r], 0) : ~0ull;
+            v[r] = nextLane64(v[r], last, lane);
         }
         size--;
     }
@@ -293,14 +293,15 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
 }
 
 // On rows of at most 64 entries, one- and two-register lists are held to 128 VGPRs (four
-// wavefronts per SIMD; a few spills) and five- to seven-register lists to 168 (three): the
-// hop is a chain of dependent memory round trips, and ef=25 / ef=100 / ef=256 / ef=350 on
-// the 1M HNSW run 1.11x / 1.17x / 1.3x / 1.2x faster than with the registers the compiler
-// would take (one-register lists held to 96 VGPRs run 1.15x slower).  Eight-register lists
-// spill too much at 168 (ef=500: 52.3 ms held, 47.7 ms not).
+// wavefronts per SIMD; a few spills) and longer lists to 168 (three): the hop is a chain
+// of dependent memory round trips, and ef=25 / ef=100 / ef=256 / ef=350 on the 1M HNSW run
+// 1.11x / 1.17x / 1.3x / 1.2x faster than with the registers the compiler would take
+// (one-register lists held to 96 VGPRs run 1.15x slower); eight-register lists (191 VGPRs
+// unbounded, two wavefronts) run ef=500 in 34.7 ms instead of 41.7 at 168 with 96 bytes
+// of spills.
 template <int WR, int MAXCH>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(MAXCH == 1 && WR <= 2 ? 4 : MAXCH == 1 && WR >= 5 && WR <= 7 ? 3 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(MAXCH == 1 && WR <= 2 ? 4 : MAXCH == 1 ? 3 : 1, 8)))
 void beamHnswKernel(HnswLaunch L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = laneId();

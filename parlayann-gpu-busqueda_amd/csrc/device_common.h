@@ -242,6 +242,13 @@ __device__ __forceinline__ uint64_t prevLane64(uint64_t v, uint64_t first, int l
     return lane == 0 ? first : (((uint64_t)hi << 32) | lo);
 }
 
+// Value of lane + 1 (lane 63 gets `last`), a whole-wave DPP shift.
+__device__ __forceinline__ uint64_t nextLane64(uint64_t v, uint64_t last, int lane) {
+    const uint32_t lo = dppMov<0x130>((uint32_t)v);
+    const uint32_t hi = dppMov<0x130>((uint32_t)(v >> 32));
+    return lane == 63 ? last : (((uint64_t)hi << 32) | lo);
+}
+
 // One compare-exchange stage at distance j < 64 inside a register: the lane keeps the
 // smaller of itself and its partner iff keepMin (one 64-bit compare).
 __device__ __forceinline__ uint64_t cmpXchg(uint64_t x, int j, bool keepMin, int lane) {
@@ -395,9 +402,12 @@ __device__ __forceinline__ bool seenFilterEnd(uint32_t* filt, const SeenProbe& s
     }
     const bool seen = part && (prev == a);
     // an HBM slot that already holds a is not written again (most entries of a long
-    // beam's rows were seen before, and each write is a partial-line HBM write)
+    // beam's rows were seen before, and each write is a partial-line HBM write).  The
+    // wave does not wait for its HBM stores here: its own later loads of a slot come after
+    // them in its memory request stream (one wave's accesses to one address keep their
+    // order, as for any plain store and load), and the next wait on a later load covers
+    // them.
     if (part && last && (!G || s.prev != a)) filtStore<G>(filt, h, a);
-    if constexpr (G) __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
     return part && !seen;
 }

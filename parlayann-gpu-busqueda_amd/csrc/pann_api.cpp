@@ -558,6 +558,19 @@ void flatBindScratch(FlatShape& sh, unsigned char* scratch) {
     }
 }
 
+// Long beams (Ls or ef of 65 and more) take their queries from a work queue (two
+// counters that the launch's last wavefront resets) instead of in turn: their search
+// lengths vary by hundreds of hops, and a wavefront that finishes early takes the next
+// query at once.  Short beams stay strided, which measured faster there.
+constexpr int kQueueMinLs = 65;
+
+int allocQueue(pann_searcher* s) {
+    hipError_t e = hipMalloc(&s->queue, 2 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(s->queue, 0, 2 * sizeof(uint32_t));
+    if (e != hipSuccess) return setError(PANN_ERR_HIP, "allocating the work queue: %s", hipGetErrorString(e));
+    return PANN_OK;
+}
+
 }  // namespace pann
 
 extern "C" {
@@ -589,7 +602,9 @@ int pann_searcher_prepare(const pann_graph* g, uint32_t nq, const pann_search_co
     flatBindScratch(sh, reinterpret_cast<unsigned char*>(s->gvis));
     s->L = sh.L;
     FlatLaunch& L = s->L;
-    r = allocOutputs(s->out, nq, (uint32_t)std::max(1, cfg->k));
+    r = cfg->Ls >= kQueueMinLs ? allocQueue(s) : PANN_OK;
+    L.queue = s->queue;
+    if (r == PANN_OK) r = allocOutputs(s->out, nq, (uint32_t)std::max(1, cfg->k));
     if (r != PANN_OK) {
         pann_searcher_free(s);
         return r;
@@ -693,7 +708,9 @@ int pann_hnsw_searcher_prepare(const pann_hnsw* h, uint32_t nq, const pann_searc
         }
     }
     H.gfilt = s->gfilt;
-    r = allocOutputs(s->out, nq, (uint32_t)std::max(1, cfg->k));
+    r = Ls >= kQueueMinLs ? allocQueue(s) : PANN_OK;
+    H.queue = s->queue;
+    if (r == PANN_OK) r = allocOutputs(s->out, nq, (uint32_t)std::max(1, cfg->k));
     if (r != PANN_OK) {
         pann_searcher_free(s);
         return r;
@@ -897,6 +914,7 @@ void pann_searcher_free(pann_searcher* s) {
     freeOutputs(s->out);
     hipFree(s->gfilt);
     hipFree(s->gvis);
+    hipFree(s->queue);
     delete s;
 }
 

@@ -48,6 +48,7 @@ struct pann_searcher {
     HnswLaunch H{};
     uint32_t* gfilt = nullptr;
     uint32_t* gvis = nullptr;  // HBM visited tables (FlatLaunch::fast -1)
+    uint32_t* queue = nullptr;  // work-queue counters of long-beam launches (else strided)
     pann_search_outputs out{};
 };
 
