@@ -118,13 +118,16 @@ def test_cut_values(gpu, small):
             _assert_same(r, o, 10, f"G=100 Ls={Ls} cut={cut}")
 
 
-def test_determinism_repeat_launch(gpu, small):
+@pytest.mark.parametrize("Ls", [50, 100, 300])
+def test_determinism_repeat_launch(gpu, small, Ls):
     """Repeated launches of one prepared searcher give identical outputs
-    (cuda/comparacion1.cu:107-109)."""
+    (cuda/comparacion1.cu:107-109).  From Ls 65 on the launches take their queries from a
+    work queue that the last wavefront resets: back-to-back launches on one stream must
+    each see it empty, and the last one must still equal the oracle."""
     from pann import search
     base, queries, graph = small
     b, q, g = search.Vectors(base), search.Vectors(queries), search.FlatGraph(graph)
-    s = search.Searcher(g, len(queries), search.SearchConfig(Ls=50, k=10))
+    s = search.Searcher(g, len(queries), search.SearchConfig(Ls=Ls, k=10))
     s.launch(b, q, g)
     search.synchronize()
     first = s.download()
@@ -134,6 +137,7 @@ def test_determinism_repeat_launch(gpu, small):
     last = s.download()
     assert (first.ids == last.ids).all() and (first.dists == last.dists).all()
     assert (first.comps == last.comps).all()
+    _assert_same(last, _oracle(base, graph, queries, Ls, 10), 10, f"repeat Ls={Ls}")
     s.free(), g.free(), q.free(), b.free()
 
 

@@ -53,6 +53,24 @@ def test_built_hnsw_parity(gpu, built, Ls, k):
     _assert_same(_run(base, h, queries, Ls, k), _oracle(base, h, queries, Ls, k), k, f"Ls={Ls} k={k}")
 
 
+@pytest.mark.parametrize("Ls", [40, 100, 300])
+def test_repeat_launch(gpu, built, Ls):
+    """Back-to-back launches of one prepared searcher (from ef 65 on they share a
+    self-resetting work queue) all equal the oracle."""
+    from pann import search
+    base, queries, h = built
+    b, q = search.Vectors(base), search.Vectors(queries)
+    idx = search.HnswIndex(h, b)
+    s = search.HnswSearcher(idx, len(queries), search.SearchConfig(Ls=Ls, k=10))
+    try:
+        for _ in range(4):
+            s.launch(q, idx)
+        search.synchronize()
+        _assert_same(s.download(), _oracle(base, h, queries, Ls, 10), 10, f"repeat Ls={Ls}")
+    finally:
+        s.free(), idx.free(), q.free(), b.free()
+
+
 @pytest.mark.parametrize("visit", [1, 2, 5, 20])
 def test_visit_limits(gpu, built, visit):
     base, queries, h = built
