@@ -312,25 +312,19 @@ __global__ __launch_bounds__(256) void bfReduceThresholdKernel(const int32_t* ac
 // next chunk's loads drain every store as well.  A row whose candidates find the pool
 // full is marked kDropped in its count (the count itself stays exact), which sends it to
 // the refilter.
-//
-// In a block, element e = 4 rb + j of the 16 lanes of group grp belongs to row
-// 16 rb + 4 grp + j: each row's candidates come from one element of one group.  So every
-// lane of a group keeps the counts of the group's 16 rows in registers (the same values in
-// its 16 lanes), a candidate's list slot is the count plus its rank among the group's
-// candidates of that element, and its pool place the wave's running pool size plus its
-// rank in the element's ballot: no atomics, and no LDS round trip before the stores.
 constexpr uint32_t kPool = 256;
 constexpr uint32_t kDropped = 0x80000000u;
 
-// Held to 168 VGPRs: the LDS of three workgroups fits a CU, and so do their registers.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bfFilterKernel(GemmArgs a) {
+__global__ __launch_bounds__(256) void bfFilterKernel(GemmArgs a) {
     __shared__ Chunk chunk[2];
     __shared__ int32_t qnLds[kGemmWaves][kRowsPerWave];
+    __shared__ uint32_t cntLds[kGemmWaves][kRowsPerWave];   // exact, | kDropped
     __shared__ uint64_t firstLds[kGemmWaves][kRowsPerWave];
     __shared__ uint32_t capLds[kGemmWaves][kRowsPerWave];
     __shared__ int32_t thrLds[kGemmWaves][kRowsPerWave];
     __shared__ uint32_t poolAt[kGemmWaves][kPool];          // row << 16 | slot
     __shared__ uint64_t poolKey[kGemmWaves][kPool];
+    __shared__ uint32_t poolN[kGemmWaves];
     const int lane = laneId(), wave = threadIdx.x >> 6;
     const int col = lane & 15, grp = lane >> 4;
     uint32_t slice, qtile;
@@ -342,93 +336,70 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
         const bool live = q0 + lane < a.nsel;
         const size_t list = (size_t)(q0 + lane) * a.slices + slice;
         qnLds[wave][lane] = live ? a.qnorm[queryOf(a, q0 + lane)] : 0;
+        cntLds[wave][lane] = 0;
         const uint64_t first = !live ? 0 : a.offs ? a.offs[list] : (uint64_t)list * a.capS;
         firstLds[wave][lane] = first;
         // slots are 16-bit in the pool; longer lists are used up to 65,535
         const uint64_t cap = !live ? 0 : a.offs ? a.offs[list + 1] - first : a.capS;
         capLds[wave][lane] = (uint32_t)min(cap, (uint64_t)0xFFFF);
         thrLds[wave][lane] = live ? a.thr[queryOf(a, q0 + lane)] : INT32_MAX;
+        if (lane == 0) poolN[wave] = 0;
     }
     __builtin_amdgcn_wave_barrier();
     // this lane's rows: e = 4 rb + j holds row 16 rb + 4 grp + j of the wavefront
     int32_t thr[16];
-    uint32_t cnt[16];  // exact candidate counts of those rows, | kDropped
 #pragma unroll
     for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            thr[rb * 4 + j] = thrLds[wave][16 * rb + 4 * grp + j];
-            cnt[rb * 4 + j] = 0;
-        }
-    const uint64_t gm = 0xFFFFull << (16 * grp);  // the lanes of this lane's group
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t capU = (uint32_t)min((uint64_t)a.capS, (uint64_t)0xFFFF);
-    uint32_t poolN = 0;  // wave-uniform
+        for (int j = 0; j < 4; j++) thr[rb * 4 + j] = thrLds[wave][16 * rb + 4 * grp + j];
     const uint32_t s0 = (uint32_t)((uint64_t)a.scanBlocks * slice / a.slices);
     const uint32_t s1 = (uint32_t)((uint64_t)a.scanBlocks * (slice + 1) / a.slices);
     scanChunks(a, chunk, s0, s1, [&](const BBlock& B, uint32_t sb) {
         v4i acc[4];
         blockProducts(A0, A1, B, acc);
-        // per element, the lanes with a candidate; a lane past the base holds none (its
-        // validity is folded into the ballots so that every lane takes the same path and the
-        // group counts and pool size stay the same in all lanes)
-        uint64_t pm[16];
+        uint64_t pm[16];  // per element, the lanes with a candidate
         uint64_t any = 0;
-        const bool bv = B.valid();
 #pragma unroll
         for (int rb = 0; rb < 4; rb++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                pm[rb * 4 + j] = __ballot(bv && acc[rb][j] > thr[rb * 4 + j]);
+                pm[rb * 4 + j] = __ballot(acc[rb][j] > thr[rb * 4 + j]);
                 any |= pm[rb * 4 + j];
             }
-        if (!any) return;
-        // rare: slots by rank within the group, pool places by rank within the wave
+        if (!any || !B.valid()) return;
+        // rare: each lane pools its candidates; list slots and pool places by LDS atomics
         const uint32_t p = baseBlockOf(a, sb) * 16 + col;
 #pragma unroll
         for (int rb = 0; rb < 4; rb++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int e = rb * 4 + j;
-                const uint64_t m = pm[e];
-                if (!m) continue;
+                if (!((pm[rb * 4 + j] >> lane) & 1)) continue;
                 const uint32_t row = 16 * rb + 4 * grp + j;
-                const bool mine = (m >> lane) & 1ull;
-                const uint32_t slot = (cnt[e] & ~kDropped) + (uint32_t)__popcll(m & gm & below);
-                cnt[e] += (uint32_t)__popcll(m & gm);
-                const uint32_t cap = a.offs ? capLds[wave][row] : capU;
-                const bool keep = mine && slot < cap;
-                const uint64_t km = __ballot(keep);
-                const uint32_t at = poolN + (uint32_t)__popcll(km & below);
-                if (keep && at < kPool) {
+                const uint32_t slot = atomicAdd(&cntLds[wave][row], 1u) & ~kDropped;
+                if (slot >= capLds[wave][row]) continue;
+                const uint32_t at = atomicAdd(&poolN[wave], 1u);
+                if (at < kPool) {
                     poolAt[wave][at] = row << 16 | slot;
                     poolKey[wave][at] = ((uint64_t)(uint32_t)(qnLds[wave][row] + B.odd() - 2 * acc[rb][j]) << 32) | p;
+                } else {
+                    atomicOr(&cntLds[wave][row], kDropped);
                 }
-                if (__ballot(keep && at >= kPool) & gm) cnt[e] |= kDropped;
-                poolN = min(kPool, poolN + (uint32_t)__popcll(km));
             }
     }, [&](bool last) {
         // between chunks: the pooled candidates to their lists when half the pool is used
         __builtin_amdgcn_wave_barrier();
-        if (!last && poolN < kPool / 2) return;
-        for (uint32_t x = lane; x < poolN; x += 64) {
+        const uint32_t pooled = min(kPool, poolN[wave]);
+        if (!last && pooled < kPool / 2) return;
+        for (uint32_t x = lane; x < pooled; x += 64) {
             const uint32_t w = poolAt[wave][x];
             a.cand[firstLds[wave][w >> 16] + (w & 0xFFFF)] = poolKey[wave][x];
         }
         __builtin_amdgcn_wave_barrier();
-        poolN = 0;
+        if (lane == 0) poolN[wave] = 0;
+        __builtin_amdgcn_wave_barrier();
     });
     __builtin_amdgcn_wave_barrier();
-    // the first lane of each group writes its group's 16 counts
-    if (col == 0) {
-#pragma unroll
-        for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t row = 16 * rb + 4 * grp + j;
-                if (q0 + row < a.nsel) a.counts[(size_t)(q0 + row) * a.slices + slice] = cnt[rb * 4 + j];
-            }
-    }
+    if (q0 + lane < a.nsel) a.counts[(size_t)(q0 + lane) * a.slices + slice] = cntLds[wave][lane];
 }
 
 // ---- pass 3: per row of the pass, sort its candidates and keep the first k ----
