@@ -101,3 +101,22 @@ def test_random_hnsw(gpu, seed, M0, M, entries, dup):
     for Ls in (8, 33, 100, 200):
         _assert_same(_run(base, h, queries, Ls, 10), _oracle(base, h, queries, Ls, 10), 10,
                      f"seed={seed} Ls={Ls}")
+
+
+def test_repeated_entry_points(gpu):
+    """An entrance set that names a node twice puts two copies of it in W (the oracle's
+    wInsert does not deduplicate); a neighbour equal to a copy must still be refused as
+    already entered (the kernel reads is_inw off W), on the upper layers and layer 0."""
+    import dataclasses
+    from pann import datagen
+    rng = np.random.default_rng(12)
+    n = 1200
+    base, queries = datagen.gmm_dataset(n, 100, dim=64, ncomp=4, seeds=(61, 62, 63))
+    h = random_hnsw(n, 64, rng, M0=24, M=10, max_level=3, num_entries=3, dup_frac=0.1)
+    e = [int(x) for x in h.entries]
+    for entries in ([e[0], e[0]], [e[0], e[1], e[0], e[2], e[1]]):
+        h2 = dataclasses.replace(h, entries=np.array(entries, np.uint32))
+        for Ls in (1, 8, 40, 120):
+            _assert_same(_run(base, h2, queries, Ls, 10), _oracle(base, h2, queries, Ls, 10), 10,
+                         f"entries={entries} Ls={Ls}")
+
