@@ -3,7 +3,8 @@
 //
 // Semantics follow the reference's restatement (cuda/busqueda_hnsw.cu:179-457):
 //   * per layer: fresh lossy visited filter (hash64 marks entry points, hash64_2 the
-//     neighbours; 2 bits above layer 0, 2*ceil(log2 ef)-2 on layer 0), fresh is_inw set;
+//     neighbours; 2 bits above layer 0, 2*ceil(log2 ef)-2 on layer 0), fresh is_inw set
+//     (the ids that entered W on the layer);
 //   * cand ordered and unique by distance, W (the workset) ordered by (distance, id);
 //   * loop: stop when cand's nearest is farther than W's farthest or the visit limit is
 //     hit; expand the nearest candidate; neighbours are accepted in row order when
@@ -11,9 +12,9 @@
 //   * the next layer starts from W's farthest element; output W[0:k] as base point ids.
 // Layout choices of this file: W and cand live in registers as wave-wide sorted lists
 // (element e = r*64 + lane), so an insertion position is one ballot popcount and an
-// insertion a lane shift; is_inw is an open-addressing hash in LDS or, for large ef, in a
-// per-wave HBM region; the sequential acceptance only visits neighbours that can still
-// pass the threshold, found by ballot.
+// insertion a whole-wave DPP lane shift; is_inw is not stored but read off W (see the
+// acceptance in expandNode); the sequential acceptance only visits neighbours that can
+// still pass the threshold, found by ballot.
 #include "device_common.h"
 
 namespace pann {

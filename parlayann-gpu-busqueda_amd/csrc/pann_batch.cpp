@@ -45,7 +45,7 @@ struct pann_batch {
     uint32_t* order = nullptr;
     uint32_t* queue = nullptr;
     uint32_t* gfilt = nullptr;
-    uint32_t* gvis = nullptr;  // HBM visited / is_inw tables of the launches that use them
+    uint32_t* gvis = nullptr;  // HBM visited tables of the flat launches that use them
     pann_search_outputs out{};
     std::vector<uint32_t> hostOrder;
     hipGraph_t graphDef = nullptr;
