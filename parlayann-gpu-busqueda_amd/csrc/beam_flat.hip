@@ -915,9 +915,27 @@ __device__ __forceinline__ UnionMarks unionMarks(uint64_t& x, uint64_t prev, int
     return m;
 }
 
-template <bool NARROW, int PREFETCH, bool NORMS>
+// Expansions of id a so far, from the register-resident forms' visited hash in LDS
+// (capped at 127, the width of the tag field that carries it).
+__device__ __forceinline__ uint32_t fastExpansions(const uint32_t* vkey, const unsigned char* vcnt,
+                                                   uint32_t a, uint32_t logCap, uint32_t vmask) {
+    uint32_t hh = visitedSlot(a, logCap);
+    while (true) {
+        const uint32_t kk = vkey[hh];
+        if (kk == a) return min(127u, (uint32_t)vcnt[hh]);
+        if (kk == kEmpty) return 0;
+        hh = (hh + 1) & vmask;
+    }
+}
+
+// ROW2: adjacency rows of up to 128 entries, read as two 64-entry halves; each half runs
+// the filter and distance steps in row order (the second half's filter sees the first's
+// slot writes), and the merge takes the candidates of both, counting earlier copies of a
+// key across the halves.
+template <bool NARROW, int PREFETCH, bool NORMS, bool ROW2 = false>
 __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, const QueryCfg& c,
                            Prof& P) {
+    static_assert(!ROW2 || PREFETCH == 0, "rows of two halves do not prefetch");
     const int lane = laneId();
     const int grp = lane >> 3, sub = lane & 7;
     const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -971,6 +989,9 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
         if (PREFETCH >= 2 && uid != predId && uid == predId2) a = aNext2;
         else if (PREFETCH == 0 || uid != predId)
             a = (uint32_t)lane < G ? *rowAt<NARROW, uint32_t>(L.adj, uid, 4 * G, 4 * (uint32_t)lane) : kEmpty;
+        uint32_t a1 = kEmpty;
+        if constexpr (ROW2)
+            if (64u + (uint32_t)lane < G) a1 = *rowAt<NARROW, uint32_t>(L.adj, uid, 4 * G, 4 * (64u + (uint32_t)lane));
 
         // visit u: its flag, and its expansion count (64 probe slots read at once)
         if (lane == u) F |= 1u;
@@ -1031,83 +1052,87 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
         const int cutoff = (fsize < Ls) ? 0x7FFFFFFF
                                         : (int)((uint32_t)readlane((int)(uint32_t)(F >> 32), fsize - 1) >> 9);
 
-        // seen filter, first half: slots and tags
-        edges += __popcll(ballot(a != kEmpty));
-        const bool part = (a != kEmpty) && (a != c.qid);
-        const SeenProbe sp = seenFilterBegin<false>(filt, tag, fmask, a, part, lane);
-        const bool spec = part && (sp.coll || sp.prev != a);
+        // one 64-entry half of the row: seen filter, vector loads, distances; the key of
+        // entry `lane` if it is a candidate below the cutoff, else kNoKey
+        auto halfKeys = [&](const uint32_t a) -> uint64_t {
+            // seen filter, first half: slots and tags
+            edges += __popcll(ballot(a != kEmpty));
+            const bool part = (a != kEmpty) && (a != c.qid);
+            const SeenProbe sp = seenFilterBegin<false>(filt, tag, fmask, a, part, lane);
+            const bool spec = part && (sp.coll || sp.prev != a);
 
-        // vectors of every entry that may pass: group g loads entry 8g + r in round r
-        const uint32_t gmask = (uint32_t)(ballot(spec) >> (8 * grp)) & 0xFFu;
-        uint32_t ids[8];
+            // vectors of every entry that may pass: group g loads entry 8g + r in round r
+            const uint32_t gmask = (uint32_t)(ballot(spec) >> (8 * grp)) & 0xFFu;
+            uint32_t ids[8];
 #pragma unroll
-        for (int r = 0; r < 8; r++) ids[r] = groupBroadcast8(a, r);
-        // Slices of entries that are not loaded keep whatever the registers hold: their
-        // distances are computed and thrown away.
-        int4 v[8];
+            for (int r = 0; r < 8; r++) ids[r] = groupBroadcast8(a, r);
+            // Slices of entries that are not loaded keep whatever the registers hold: their
+            // distances are computed and thrown away.
+            int4 v[8];
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-            // an empty asm hands the compiler registers it need not initialise
-            asm("" : "=v"(v[r].x), "=v"(v[r].y), "=v"(v[r].z), "=v"(v[r].w));
-            if ((gmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
-        }
-
-        // expansions of the id so far (decide the visited flags of entering copies)
-        uint32_t cnt = 0;
-        if (spec) {
-            uint32_t hh = visitedSlot(a, logCap);
-            while (true) {
-                const uint32_t kk = vkey[hh];
-                if (kk == a) {
-                    cnt = min(127u, (uint32_t)vcnt[hh]);
-                    break;
-                }
-                if (kk == kEmpty) break;
-                hh = (hh + 1) & vmask;
+            for (int r = 0; r < 8; r++) {
+                // an empty asm hands the compiler registers it need not initialise
+                asm("" : "=v"(v[r].x), "=v"(v[r].y), "=v"(v[r].z), "=v"(v[r].w));
+                if ((gmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
             }
-        }
 
-        // seen filter, second half: in-row collisions, slot writes
-        const bool keep = seenFilterEnd<false>(filt, sp, a, part, lane, lanesBelow);
-        const uint64_t keepM = ballot(keep);
-        comps += __popcll(keepM);
-        // A lane that won its byte tag but shares the slot with an earlier lane was not
-        // speculated when the slot held its id before the row; load those now.
-        const uint64_t late = keepM & ~ballot(spec);
-        if (late) {
-            const uint32_t lmask = (uint32_t)(late >> (8 * grp)) & 0xFFu;
+            // expansions of the id so far (decide the visited flags of entering copies)
+            uint32_t cnt = spec ? fastExpansions(vkey, vcnt, a, logCap, vmask) : 0u;
+
+            // seen filter, second half: in-row collisions, slot writes
+            const bool keep = seenFilterEnd<false>(filt, sp, a, part, lane, lanesBelow);
+            const uint64_t keepM = ballot(keep);
+            comps += __popcll(keepM);
+            // A lane that won its byte tag but shares the slot with an earlier lane was not
+            // speculated when the slot held its id before the row; load those now.
+            // Their ids held the slot before the row, so they may have been expanded: their
+            // expansion counts are read now too.
+            const uint64_t late = keepM & ~ballot(spec);
+            if (late) {
+                const uint32_t lmask = (uint32_t)(late >> (8 * grp)) & 0xFFu;
+#pragma unroll
+                for (int r = 0; r < 8; r++)
+                    if ((lmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
+                if ((late >> lane) & 1ull) cnt = fastExpansions(vkey, vcnt, a, logCap, vmask);
+            }
+            P.mark(kProfFilter);
+
+            int part8[8];
 #pragma unroll
             for (int r = 0; r < 8; r++)
-                if ((lmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
-        }
-        P.mark(kProfFilter);
-
-        int part8[8];
-#pragma unroll
-        for (int r = 0; r < 8; r++)
-            part8[r] = NORMS ? slicePartialN(v[r], qv, normMask) : slicePartial(v[r], qv);
-        // p.p is recomputed from the row: a stored norm would cost one more random
-        // cache line per candidate
-        const int d = qq + transposeReduce8(part8, lane);  // distance of entry `lane`
-        const uint64_t C = (keep && d < cutoff) ? tagKey(d, a, kCand | cnt) : kNoKey;
-        P.mark(kProfDist);
+                part8[r] = NORMS ? slicePartialN(v[r], qv, normMask) : slicePartial(v[r], qv);
+            // p.p is recomputed from the row: a stored norm would cost one more random
+            // cache line per candidate
+            const int d = qq + transposeReduce8(part8, lane);  // distance of entry `lane`
+            const uint64_t key = (keep && d < cutoff) ? tagKey(d, a, kCand | cnt) : kNoKey;
+            P.mark(kProfDist);
+            return key;
+        };
+        const uint64_t C = halfKeys(a);
+        uint64_t C1 = kNoKey;
+        if constexpr (ROW2) C1 = halfKeys(a1);
 
         const uint64_t candM = ballot(C != kNoKey);
-        const int nc = __popcll(candM);
+        const uint64_t candM1 = ROW2 ? ballot(C1 != kNoKey) : 0ull;
+        const int nc = __popcll(candM) + __popcll(candM1);
         P.add(kProfCands, (unsigned long long)nc);
         if (nc == 0) P.count(kProfNoCands);
         if (nc > 0 && nc <= 8) P.count(kProfSmall);
 
         bool changed = false;
-        if (nc > 0 && nc <= kInsertMax) {
+        // candidates from both halves of a row go through the insertion, whatever their number
+        const bool bothHalves = ROW2 && candM != 0 && candM1 != 0;
+        if (nc > 0 && (nc <= kInsertMax || bothHalves)) {
             // insert each at its upper bound (equal keys: frontier copies, then the
             // candidate copies already inserted), trimming to Ls as we go
             int fs = fsize;
-            uint64_t mm = candM;
+            for (int h = 0; h < (ROW2 ? 2 : 1); h++) {
+            const uint64_t Ch = h ? C1 : C;
+            uint64_t mm = h ? candM1 : candM;
             while (mm) {
                 const int j = __ffsll((unsigned long long)mm) - 1;
                 mm &= mm - 1;
-                const uint64_t x = readlane64(C, j);
+                const uint64_t x = readlane64(Ch, j);
                 const uint64_t xmin = x & ~kTagMask, xmax = x | kTagMask;
                 const int pos = __popcll(ballot(F <= xmax));
                 if (pos >= Ls) continue;
@@ -1119,7 +1144,10 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
                 uint64_t xe = x;
                 if (pos != lower) {
                     const int cF = __popcll(ballot(F >= xmin && F <= xmax && !(F & kCand)));
-                    const int rep = __popcll(ballot(C == x) & ((1ull << j) - 1));
+                    // earlier copies: in this half below lane j, and (second half) all of
+                    // the first half's
+                    const int rep = __popcll(ballot(Ch == x) & ((1ull << j) - 1)) +
+                                    (h ? __popcll(ballot(C == x)) : 0);
                     if (rep < cF) continue;
                     xe = xmin | kCand | (uint32_t)max(0, (int)(x & 0x7Fu) - rep);
                 }
@@ -1132,6 +1160,7 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
                 fs = min(Ls, fs + 1);
                 changed = true;
             }
+            }
             if (changed) {
                 const uint32_t vis = (F & kCand) ? ((F & 0x7Fu) ? 1u : 0u) : (uint32_t)(F & 1u);
                 F = lane < fs ? ((F & ~kTagMask) | ((uint64_t)lane << 1) | vis) : kNoKey;
@@ -1139,7 +1168,8 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
             }
             P.mark(kProfMerge);
         } else if (nc > kInsertMax) {
-            const uint64_t Cd = sortWave<true>(C, lane);
+            // (rows of two halves: all candidates lie in one half here)
+            const uint64_t Cd = sortWave<true>(ROW2 && candM == 0 ? C1 : C, lane);
             P.mark(kProfSort);
 
             // bitonic merge of frontier (ascending) and candidates (descending)
@@ -1345,19 +1375,7 @@ __device__ void searchFast2(const FlatLaunch& L, unsigned char* lds, uint32_t q,
             if ((gmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
         }
 
-        uint32_t cnt = 0;
-        if (spec) {
-            uint32_t hh = visitedSlot(a, logCap);
-            while (true) {
-                const uint32_t kk = vkey[hh];
-                if (kk == a) {
-                    cnt = min(127u, (uint32_t)vcnt[hh]);
-                    break;
-                }
-                if (kk == kEmpty) break;
-                hh = (hh + 1) & vmask;
-            }
-        }
+        uint32_t cnt = spec ? fastExpansions(vkey, vcnt, a, logCap, vmask) : 0u;
 
         const bool keep = seenFilterEnd<true>(filt, sp, a, part, lane, lanesBelow);
         const uint64_t keepM = ballot(keep);
@@ -1368,6 +1386,8 @@ __device__ void searchFast2(const FlatLaunch& L, unsigned char* lds, uint32_t q,
 #pragma unroll
             for (int r = 0; r < 8; r++)
                 if ((lmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
+            // ids that held their slot before the row: read their expansion counts too
+            if ((late >> lane) & 1ull) cnt = fastExpansions(vkey, vcnt, a, logCap, vmask);
         }
         P.mark(kProfFilter);
 
@@ -1559,7 +1579,13 @@ __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
         }
         makeUniform(c);
         const int fb = flatFilterBits(c.Ls);
-        if constexpr (FAST >= 13) {
+        if constexpr (FAST >= 17) {
+            if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride) {
+                rejectQuery(L, q, lane);
+            } else {
+                searchFast<((FAST - 17) & 1) != 0, 0, ((FAST - 17) & 2) != 0, true>(L, lds, q, c, P);
+            }
+        } else if constexpr (FAST >= 13) {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride) {
                 rejectQuery(L, q, lane);
             } else {
@@ -1615,7 +1641,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MAXCH == 1 
 // dependent gathers and LDS round trips, and the extra wavefront hides them.
 template <int FAST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void beamFlatFastKernel(FlatLaunch L) {
-    drainQueries<1, FAST>(L);
+    drainQueries<(FAST >= 17 ? 2 : 1), FAST>(L);
 }
 
 // The two-register form: its LDS (6-11 KB per wavefront) allows about four wavefronts
@@ -1750,6 +1776,18 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
         return hipGetLastError();
     }
     if (L.fast) {
+        if (L.fast >= 17) {
+            // rows of up to 128 entries in two halves
+            if (maxch != 2 || L.LsMax > kFastLsMax) return hipErrorInvalidValue;
+            switch (L.fast + (L.normsInRow ? 2 : 0)) {
+                case 17: hipLaunchKernelGGL((beamFlatFastKernel<17>), grid, block, lds, stream, L); break;
+                case 18: hipLaunchKernelGGL((beamFlatFastKernel<18>), grid, block, lds, stream, L); break;
+                case 19: hipLaunchKernelGGL((beamFlatFastKernel<19>), grid, block, lds, stream, L); break;
+                case 20: hipLaunchKernelGGL((beamFlatFastKernel<20>), grid, block, lds, stream, L); break;
+                default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        }
         if (maxch != 1) return hipErrorInvalidValue;
         if (L.fast >= 13) {
             if (L.LsMax > kFastLsMax) return hipErrorInvalidValue;
@@ -1817,6 +1855,11 @@ hipError_t beamFlatOccupancy(int maxch, int fast, int wavesPerBlock, size_t ldsB
             case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatWideKernel<4>, threads, ldsBytes);
             default: return hipErrorInvalidValue;
         }
+    }
+    if (fast >= 17) {
+        if (maxch != 2) return hipErrorInvalidValue;
+        return fast == 17 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFastKernel<17>, threads, ldsBytes)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFastKernel<18>, threads, ldsBytes);
     }
     if (fast >= 13) {
         if (maxch != 1) return hipErrorInvalidValue;

@@ -151,8 +151,10 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int*
     const int fbLds = std::min(c.lds_filter_bits ? c.lds_filter_bits : kFilterBitsLds, fbMax);
     const uint32_t derived = flatVisitedCap(LsHi);
     const uint32_t capV = !c.visited_slots ? derived : top ? c.visited_slots : std::min(c.visited_slots, derived);
-    // one frontier key per lane (filter in LDS), or two (64 < Ls <= 128, filter in HBM)
-    const bool fast = b->maxch == 1 && LsHi <= kFastLsMax && fbLds >= kFastFilterBits;
+    // one frontier key per lane (filter in LDS; rows of up to 128 entries in two halves),
+    // or two (64 < Ls <= 128, rows of at most 64, filter in HBM)
+    const bool fast = b->maxch <= 2 && LsHi <= kFastLsMax && fbLds >= kFastFilterBits;
+    const bool row2 = fast && b->maxch == 2;
     const bool fast2 = b->maxch == 1 && !fast && LsHi <= kFast2LsMax;
     const bool hbmVisited = !fast && !fast2 && capV >= kVisitedHbmMinCap;
     // a general launch whose shortest beam already needs a larger filter keeps none in LDS
@@ -160,7 +162,8 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int*
     const int mode = fastMode(b->graph->n, b->graph->G);
     P->LsLo = LsLo;
     P->LsHi = LsHi;
-    P->fastKind = fast ? (c.prefetch == 2 ? 12 + mode : mode + (c.prefetch ? 2 : 0))
+    P->fastKind = row2 ? 16 + mode
+                : fast ? (c.prefetch == 2 ? 12 + mode : mode + (c.prefetch ? 2 : 0))
                        : fast2 ? 8 + mode : hbmVisited ? -1 : 0;
     // several wavefronts per query: the general form only
     const int wpq = (!fast && !fast2 && c.waves_per_query > 1) ? (int)c.waves_per_query : 1;
@@ -200,7 +203,7 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int*
     if (top) {
         b->cfg.lds_filter_bits = fbLds;
         b->cfg.visited_slots = capV;
-        if (!fast) b->cfg.prefetch = 0;
+        if (!fast || row2) b->cfg.prefetch = 0;
     }
     return PANN_OK;
 }
@@ -298,7 +301,8 @@ int pann_batch_prepare_flat(const pann_vectors* base, const pann_graph* g, uint3
     b->dim = base->dim;
     b->maxch = g->G <= 64 ? 1 : g->G <= 128 ? 2 : 4;
     const int LsMax = cfg->max_Ls;
-    if (LsMax > kFastLsMax) b->cfg.prefetch = 0;  // only the one-key form prefetches
+    // only the one-key form over rows of at most 64 entries prefetches
+    if (LsMax > kFastLsMax || b->maxch > 1) b->cfg.prefetch = 0;
     std::vector<int> his;
     for (int hi : {kFastLsMax, kFast2LsMax})
         if (hi < LsMax) his.push_back(hi);

@@ -165,7 +165,9 @@ struct FlatLaunch {
     int32_t fast;              // register-resident kernel (layout flatFastLdsLayout): 1, or
                                // 2 when base and adjacency are below 4 GiB; +2 with next-row
                                // prefetch (the launcher adds 4 for normsInRow); 13 or 14 with
-                               // the two-row prefetch (the launcher adds 2).  Two-register
+                               // the two-row prefetch (the launcher adds 2); 17 or 18 over
+                               // rows of 65 to 128 entries read in two halves (the launcher
+                               // adds 2).  Two-register
                                // form (flatFast2LdsLayout, filter in gfilt): 9, or 10 below
                                // 4 GiB (the launcher adds 2 for normsInRow).  0: general;
                                // -1: general with the visited table in HBM (gvkey/gvcnt)
