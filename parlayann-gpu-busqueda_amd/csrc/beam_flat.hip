@@ -1052,9 +1052,67 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
         const int cutoff = (fsize < Ls) ? 0x7FFFFFFF
                                         : (int)((uint32_t)readlane((int)(uint32_t)(F >> 32), fsize - 1) >> 9);
 
-        // one 64-entry half of the row: seen filter, vector loads, distances; the key of
-        // entry `lane` if it is a candidate below the cutoff, else kNoKey
-        auto halfKeys = [&](const uint32_t a) -> uint64_t {
+        uint64_t C, C1 = kNoKey;
+        if constexpr (ROW2) {
+            // one 64-entry half of the row: seen filter, vector loads, distances; the key
+            // of entry `lane` if it is a candidate below the cutoff, else kNoKey
+            auto halfKeys = [&](const uint32_t a) -> uint64_t {
+                // seen filter, first half: slots and tags
+                edges += __popcll(ballot(a != kEmpty));
+                const bool part = (a != kEmpty) && (a != c.qid);
+                const SeenProbe sp = seenFilterBegin<false>(filt, tag, fmask, a, part, lane);
+                const bool spec = part && (sp.coll || sp.prev != a);
+
+                // vectors of every entry that may pass: group g loads entry 8g + r in round r
+                const uint32_t gmask = (uint32_t)(ballot(spec) >> (8 * grp)) & 0xFFu;
+                uint32_t ids[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) ids[r] = groupBroadcast8(a, r);
+                // Slices of entries that are not loaded keep whatever the registers hold: their
+                // distances are computed and thrown away.
+                int4 v[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    // an empty asm hands the compiler registers it need not initialise
+                    asm("" : "=v"(v[r].x), "=v"(v[r].y), "=v"(v[r].z), "=v"(v[r].w));
+                    if ((gmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
+                }
+
+                // expansions of the id so far (decide the visited flags of entering copies)
+                uint32_t cnt = spec ? fastExpansions(vkey, vcnt, a, logCap, vmask) : 0u;
+
+                // seen filter, second half: in-row collisions, slot writes
+                const bool keep = seenFilterEnd<false>(filt, sp, a, part, lane, lanesBelow);
+                const uint64_t keepM = ballot(keep);
+                comps += __popcll(keepM);
+                // A lane that won its byte tag but shares the slot with an earlier lane was not
+                // speculated when the slot held its id before the row; load those now.
+                // Their ids held the slot before the row, so they may have been expanded: their
+                // expansion counts are read now too.
+                const uint64_t late = keepM & ~ballot(spec);
+                if (late) {
+                    const uint32_t lmask = (uint32_t)(late >> (8 * grp)) & 0xFFu;
+#pragma unroll
+                    for (int r = 0; r < 8; r++)
+                        if ((lmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
+                    if ((late >> lane) & 1ull) cnt = fastExpansions(vkey, vcnt, a, logCap, vmask);
+                }
+                P.mark(kProfFilter);
+
+                int part8[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++)
+                    part8[r] = NORMS ? slicePartialN(v[r], qv, normMask) : slicePartial(v[r], qv);
+                // p.p is recomputed from the row: a stored norm would cost one more random
+                // cache line per candidate
+                const int d = qq + transposeReduce8(part8, lane);  // distance of entry `lane`
+                const uint64_t key = (keep && d < cutoff) ? tagKey(d, a, kCand | cnt) : kNoKey;
+                P.mark(kProfDist);
+                return key;
+            };
+            C = halfKeys(a);
+            C1 = halfKeys(a1);
+        } else {
             // seen filter, first half: slots and tags
             edges += __popcll(ballot(a != kEmpty));
             const bool part = (a != kEmpty) && (a != c.qid);
@@ -1104,13 +1162,9 @@ __device__ void searchFast(const FlatLaunch& L, unsigned char* lds, uint32_t q, 
             // p.p is recomputed from the row: a stored norm would cost one more random
             // cache line per candidate
             const int d = qq + transposeReduce8(part8, lane);  // distance of entry `lane`
-            const uint64_t key = (keep && d < cutoff) ? tagKey(d, a, kCand | cnt) : kNoKey;
+            C = (keep && d < cutoff) ? tagKey(d, a, kCand | cnt) : kNoKey;
             P.mark(kProfDist);
-            return key;
-        };
-        const uint64_t C = halfKeys(a);
-        uint64_t C1 = kNoKey;
-        if constexpr (ROW2) C1 = halfKeys(a1);
+        }
 
         const uint64_t candM = ballot(C != kNoKey);
         const uint64_t candM1 = ROW2 ? ballot(C1 != kNoKey) : 0ull;
