@@ -164,8 +164,9 @@ def test_autotune_returns_clean_config(gpu, small):
 def test_register_resident_batch_forms(gpu, w1, monkeypatch, prefetch, wide, max_Ls):
     """Batches with max_Ls <= 64 (one frontier key per lane) or <= 128 (two keys per lane,
     seen filter in HBM) run the register-resident kernels: with and without the next-row
-    prefetch, with 32-bit and 64-bit addressing, on uneven Ls / k / visit limits and on a
-    graph with duplicate edges."""
+    prefetch, with 32-bit and 64-bit addressing, on uneven Ls / k / visit limits, on a
+    graph with duplicate edges, and on rows of up to 100 entries (read in two halves, no
+    prefetch)."""
     from oracle import oracle as O
     from pann import datagen, search
     from conftest import random_graph
@@ -176,11 +177,12 @@ def test_register_resident_batch_forms(gpu, w1, monkeypatch, prefetch, wide, max
     cases = [(base, queries, graph)]
     b2, q2 = datagen.gmm_dataset(3000, 150, dim=100, ncomp=6, seeds=(21, 22, 23))
     cases.append((b2, q2, random_graph(3000, 64, np.random.default_rng(64), dup_frac=0.2, min_degree=21)))
+    cases.append((b2, q2, random_graph(3000, 100, np.random.default_rng(65), dup_frac=0.2, min_degree=40)))
     for base, queries, graph in cases:
         params = _mixed_params(len(queries), len(base), rng, ls_lo=1, ls_hi=max_Ls, ks=(1, 10, 50))
         b, g = search.Vectors(base), search.FlatGraph(graph)
         batch = search.Batch(len(queries), max_Ls, 50, base=b, graph=g, prefetch=prefetch)
-        assert batch.config()[0]["prefetch"] == (prefetch if max_Ls <= 64 else 0)
+        assert batch.config()[0]["prefetch"] == (prefetch if max_Ls <= 64 and graph.max_degree <= 64 else 0)
         batch.set_queries(queries, params)
         batch.launch()
         search.synchronize()
@@ -232,7 +234,8 @@ def test_ls_class_launches(gpu, w1):
     o = O.beam_flat(base, graph, queries[:nq], _oracle_params(params))
     _check_against_oracle(r, o, params, "recaptured")
     batch.free(), st.free(), g.free(), b.free()
-    # rows of up to 100 entries: every class runs the general kernel
+    # rows of up to 100 entries: beams up to 64 run the register-resident kernel over rows
+    # read in two halves, the other classes the general kernel
     b2, q2 = datagen.gmm_dataset(3000, 5000, dim=100, ncomp=6, seeds=(24, 25, 26))
     g2 = random_graph(3000, 100, np.random.default_rng(100), dup_frac=0.1, min_degree=30)
     params = _mixed_params(len(q2), len(b2), rng, ls_lo=5, ls_hi=250, ks=(1, 10, 20))
