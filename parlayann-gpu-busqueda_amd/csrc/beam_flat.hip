@@ -1319,7 +1319,7 @@ __device__ __forceinline__ uint64_t frontierKey(uint64_t F0, uint64_t F1, int p)
 // searchFast; the union of more than kInsertMax candidates merges the descending
 // candidates with F0, then the upper half of that with F1: every element of the first
 // half is <= every element of F1, so the two merges give the sorted union.
-template <bool NARROW, bool NORMS>
+template <bool NARROW, bool NORMS, bool ROW2 = false>
 __device__ void searchFast2(const FlatLaunch& L, unsigned char* lds, uint32_t q, const QueryCfg& c,
                             uint32_t* filt, Prof& P) {
     const int lane = laneId();
@@ -1370,6 +1370,9 @@ __device__ void searchFast2(const FlatLaunch& L, unsigned char* lds, uint32_t q,
         const uint32_t uid = keyId(frontierKey(F0, F1, u));
         const uint32_t a =
             (uint32_t)lane < G ? *rowAt<NARROW, uint32_t>(L.adj, uid, 4 * G, 4 * (uint32_t)lane) : kEmpty;
+        uint32_t a1 = kEmpty;
+        if constexpr (ROW2)
+            if (64u + (uint32_t)lane < G) a1 = *rowAt<NARROW, uint32_t>(L.adj, uid, 4 * G, 4 * (64u + (uint32_t)lane));
 
         if (u < 64) {
             if (lane == u) F0 |= 1u;
@@ -1413,56 +1416,110 @@ __device__ void searchFast2(const FlatLaunch& L, unsigned char* lds, uint32_t q,
 
         const int cutoff = (fsize < Ls) ? 0x7FFFFFFF : keyDist(frontierKey(F0, F1, fsize - 1));
 
-        edges += __popcll(ballot(a != kEmpty));
-        const bool part = (a != kEmpty) && (a != c.qid);
-        const SeenProbe sp = seenFilterBegin<true>(filt, tag, fmask, a, part, lane);
-        const bool spec = part && (sp.coll || sp.prev != a);
+        uint64_t C, C1 = kNoKey;
+        if constexpr (ROW2) {
+            // one 64-entry half of the row (rows of 65 to 128 entries)
+            auto halfKeys = [&](const uint32_t a) -> uint64_t {
+                edges += __popcll(ballot(a != kEmpty));
+                const bool part = (a != kEmpty) && (a != c.qid);
+                const SeenProbe sp = seenFilterBegin<true>(filt, tag, fmask, a, part, lane);
+                const bool spec = part && (sp.coll || sp.prev != a);
 
-        const uint32_t gmask = (uint32_t)(ballot(spec) >> (8 * grp)) & 0xFFu;
-        uint32_t ids[8];
+                const uint32_t gmask = (uint32_t)(ballot(spec) >> (8 * grp)) & 0xFFu;
+                uint32_t ids[8];
 #pragma unroll
-        for (int r = 0; r < 8; r++) ids[r] = groupBroadcast8(a, r);
-        int4 v[8];
+                for (int r = 0; r < 8; r++) ids[r] = groupBroadcast8(a, r);
+                int4 v[8];
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-            asm("" : "=v"(v[r].x), "=v"(v[r].y), "=v"(v[r].z), "=v"(v[r].w));
-            if ((gmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
-        }
+                for (int r = 0; r < 8; r++) {
+                    asm("" : "=v"(v[r].x), "=v"(v[r].y), "=v"(v[r].z), "=v"(v[r].w));
+                    if ((gmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
+                }
 
-        uint32_t cnt = spec ? fastExpansions(vkey, vcnt, a, logCap, vmask) : 0u;
+                uint32_t cnt = spec ? fastExpansions(vkey, vcnt, a, logCap, vmask) : 0u;
 
-        const bool keep = seenFilterEnd<true>(filt, sp, a, part, lane, lanesBelow);
-        const uint64_t keepM = ballot(keep);
-        comps += __popcll(keepM);
-        const uint64_t late = keepM & ~ballot(spec);
-        if (late) {
-            const uint32_t lmask = (uint32_t)(late >> (8 * grp)) & 0xFFu;
+                const bool keep = seenFilterEnd<true>(filt, sp, a, part, lane, lanesBelow);
+                const uint64_t keepM = ballot(keep);
+                comps += __popcll(keepM);
+                const uint64_t late = keepM & ~ballot(spec);
+                if (late) {
+                    const uint32_t lmask = (uint32_t)(late >> (8 * grp)) & 0xFFu;
+#pragma unroll
+                    for (int r = 0; r < 8; r++)
+                        if ((lmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
+                    // ids that held their slot before the row: read their expansion counts too
+                    if ((late >> lane) & 1ull) cnt = fastExpansions(vkey, vcnt, a, logCap, vmask);
+                }
+                P.mark(kProfFilter);
+
+                int part8[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++)
+                    part8[r] = NORMS ? slicePartialN(v[r], qv, normMask) : slicePartial(v[r], qv);
+                const int d = qq + transposeReduce8(part8, lane);
+                const uint64_t key = (keep && d < cutoff) ? tagKey(d, a, kCand | cnt) : kNoKey;
+                P.mark(kProfDist);
+                return key;
+            };
+            C = halfKeys(a);
+            C1 = halfKeys(a1);
+        } else {
+            edges += __popcll(ballot(a != kEmpty));
+            const bool part = (a != kEmpty) && (a != c.qid);
+            const SeenProbe sp = seenFilterBegin<true>(filt, tag, fmask, a, part, lane);
+            const bool spec = part && (sp.coll || sp.prev != a);
+
+            const uint32_t gmask = (uint32_t)(ballot(spec) >> (8 * grp)) & 0xFFu;
+            uint32_t ids[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) ids[r] = groupBroadcast8(a, r);
+            int4 v[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                asm("" : "=v"(v[r].x), "=v"(v[r].y), "=v"(v[r].z), "=v"(v[r].w));
+                if ((gmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
+            }
+
+            uint32_t cnt = spec ? fastExpansions(vkey, vcnt, a, logCap, vmask) : 0u;
+
+            const bool keep = seenFilterEnd<true>(filt, sp, a, part, lane, lanesBelow);
+            const uint64_t keepM = ballot(keep);
+            comps += __popcll(keepM);
+            const uint64_t late = keepM & ~ballot(spec);
+            if (late) {
+                const uint32_t lmask = (uint32_t)(late >> (8 * grp)) & 0xFFu;
+#pragma unroll
+                for (int r = 0; r < 8; r++)
+                    if ((lmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
+                // ids that held their slot before the row: read their expansion counts too
+                if ((late >> lane) & 1ull) cnt = fastExpansions(vkey, vcnt, a, logCap, vmask);
+            }
+            P.mark(kProfFilter);
+
+            int part8[8];
 #pragma unroll
             for (int r = 0; r < 8; r++)
-                if ((lmask >> r) & 1u) v[r] = *rowAt<NARROW, int4>(L.base, ids[r], kVecBytes, sub * 16);
-            // ids that held their slot before the row: read their expansion counts too
-            if ((late >> lane) & 1ull) cnt = fastExpansions(vkey, vcnt, a, logCap, vmask);
+                part8[r] = NORMS ? slicePartialN(v[r], qv, normMask) : slicePartial(v[r], qv);
+            const int d = qq + transposeReduce8(part8, lane);
+            C = (keep && d < cutoff) ? tagKey(d, a, kCand | cnt) : kNoKey;
+            P.mark(kProfDist);
         }
-        P.mark(kProfFilter);
-
-        int part8[8];
-#pragma unroll
-        for (int r = 0; r < 8; r++)
-            part8[r] = NORMS ? slicePartialN(v[r], qv, normMask) : slicePartial(v[r], qv);
-        const int d = qq + transposeReduce8(part8, lane);
-        const uint64_t C = (keep && d < cutoff) ? tagKey(d, a, kCand | cnt) : kNoKey;
-        P.mark(kProfDist);
 
         const uint64_t candM = ballot(C != kNoKey);
-        const int nc = __popcll(candM);
+        const uint64_t candM1 = ROW2 ? ballot(C1 != kNoKey) : 0ull;
+        const int nc = __popcll(candM) + __popcll(candM1);
         bool changed = false;
-        if (nc > 0 && nc <= kInsertMax) {
+        // candidates from both halves of a row go through the insertion, whatever their number
+        const bool bothHalves = ROW2 && candM != 0 && candM1 != 0;
+        if (nc > 0 && (nc <= kInsertMax || bothHalves)) {
             int fs = fsize;
-            uint64_t mm = candM;
+            for (int h = 0; h < (ROW2 ? 2 : 1); h++) {
+            const uint64_t Ch = h ? C1 : C;
+            uint64_t mm = h ? candM1 : candM;
             while (mm) {
                 const int j = __ffsll((unsigned long long)mm) - 1;
                 mm &= mm - 1;
-                const uint64_t x = readlane64(C, j);
+                const uint64_t x = readlane64(Ch, j);
                 const uint64_t xmin = x & ~kTagMask, xmax = x | kTagMask;
                 const int pos = __popcll(ballot(F0 <= xmax)) + __popcll(ballot(F1 <= xmax));
                 if (pos >= Ls) continue;
@@ -1471,7 +1528,10 @@ __device__ void searchFast2(const FlatLaunch& L, unsigned char* lds, uint32_t q,
                 if (pos != lower) {
                     const int cF = __popcll(ballot(F0 >= xmin && F0 <= xmax && !(F0 & kCand))) +
                                    __popcll(ballot(F1 >= xmin && F1 <= xmax && !(F1 & kCand)));
-                    const int rep = __popcll(ballot(C == x) & ((1ull << j) - 1));
+                    // earlier copies: in this half below lane j, and (second half) all of
+                    // the first half's
+                    const int rep = __popcll(ballot(Ch == x) & ((1ull << j) - 1)) +
+                                    (h ? __popcll(ballot(C == x)) : 0);
                     if (rep < cF) continue;
                     xe = xmin | kCand | (uint32_t)max(0, (int)(x & 0x7Fu) - rep);
                 }
@@ -1494,6 +1554,7 @@ __device__ void searchFast2(const FlatLaunch& L, unsigned char* lds, uint32_t q,
                 fs = min(Ls, fs + 1);
                 changed = true;
             }
+            }
             if (changed) {
                 const uint32_t vis0 = (F0 & kCand) ? ((F0 & 0x7Fu) ? 1u : 0u) : (uint32_t)(F0 & 1u);
                 const uint32_t vis1 = (F1 & kCand) ? ((F1 & 0x7Fu) ? 1u : 0u) : (uint32_t)(F1 & 1u);
@@ -1503,7 +1564,8 @@ __device__ void searchFast2(const FlatLaunch& L, unsigned char* lds, uint32_t q,
             }
             P.mark(kProfMerge);
         } else if (nc > kInsertMax) {
-            const uint64_t Cd = sortWave<true>(C, lane);
+            // (rows of two halves: all candidates lie in one half here)
+            const uint64_t Cd = sortWave<true>(ROW2 && candM == 0 ? C1 : C, lane);
             P.mark(kProfSort);
             // F0 (ascending) with the candidates (descending), then the upper half of that
             // (reversed to descending) with F1
@@ -1633,7 +1695,13 @@ __device__ __forceinline__ void drainQueries(const FlatLaunch& L) {
         }
         makeUniform(c);
         const int fb = flatFilterBits(c.Ls);
-        if constexpr (FAST >= 17) {
+        if constexpr (FAST >= 21) {
+            if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride || fb > L.gbitsMax) {
+                rejectQuery(L, q, lane);
+            } else {
+                searchFast2<((FAST - 21) & 1) != 0, ((FAST - 21) & 2) != 0, true>(L, lds, q, c, gfilt, P);
+            }
+        } else if constexpr (FAST >= 17) {
             if (c.Ls < 1 || c.Ls > L.LsMax || c.k < 0 || (uint32_t)c.k > L.outStride) {
                 rejectQuery(L, q, lane);
             } else {
@@ -1702,7 +1770,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
 // per SIMD, so it is held to 128 VGPRs.
 template <int FAST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void beamFlatFast2Kernel(FlatLaunch L) {
-    drainQueries<1, FAST>(L);
+    drainQueries<(FAST >= 21 ? 2 : 1), FAST>(L);
 }
 
 // The cooperative general form: blocks of WPQ wavefronts, one query per block at a time
@@ -1830,6 +1898,18 @@ hipError_t launchBeamFlat(const FlatLaunch& L, int maxch, int wavesPerBlock, int
         return hipGetLastError();
     }
     if (L.fast) {
+        if (L.fast >= 21) {
+            // two registers per lane, rows of up to 128 entries in two halves
+            if (maxch != 2 || L.LsMax > kFast2LsMax || L.gfilt == nullptr) return hipErrorInvalidValue;
+            switch (L.fast + (L.normsInRow ? 2 : 0)) {
+                case 21: hipLaunchKernelGGL((beamFlatFast2Kernel<21>), grid, block, lds, stream, L); break;
+                case 22: hipLaunchKernelGGL((beamFlatFast2Kernel<22>), grid, block, lds, stream, L); break;
+                case 23: hipLaunchKernelGGL((beamFlatFast2Kernel<23>), grid, block, lds, stream, L); break;
+                case 24: hipLaunchKernelGGL((beamFlatFast2Kernel<24>), grid, block, lds, stream, L); break;
+                default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        }
         if (L.fast >= 17) {
             // rows of up to 128 entries in two halves
             if (maxch != 2 || L.LsMax > kFastLsMax) return hipErrorInvalidValue;
@@ -1909,6 +1989,11 @@ hipError_t beamFlatOccupancy(int maxch, int fast, int wavesPerBlock, size_t ldsB
             case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatWideKernel<4>, threads, ldsBytes);
             default: return hipErrorInvalidValue;
         }
+    }
+    if (fast >= 21) {
+        if (maxch != 2) return hipErrorInvalidValue;
+        return fast == 21 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFast2Kernel<21>, threads, ldsBytes)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, beamFlatFast2Kernel<22>, threads, ldsBytes);
     }
     if (fast >= 17) {
         if (maxch != 2) return hipErrorInvalidValue;

@@ -497,14 +497,14 @@ int flatShape(const pann_graph* g, uint32_t nq, const pann_search_config* cfg, F
     const int Ls = cfg->Ls;
     sh.fbits = flatFilterBits(Ls);
     const uint32_t capV = flatVisitedCap(Ls);
-    // one frontier key per lane (rows of up to 128 entries in two halves), or two (rows of at
-    // most 64, filter in HBM)
+    // one frontier key per lane, or two (64 < Ls <= 128, filter in HBM); rows of 65 to 128
+    // entries are read in two halves
     const bool fast = sh.maxch <= 2 && Ls <= kFastLsMax;
-    const bool fast2 = sh.maxch == 1 && !fast && Ls <= kFast2LsMax;
+    const bool fast2 = sh.maxch <= 2 && !fast && Ls <= kFast2LsMax;
     sh.ldsFilter = sh.fbits <= kFilterBitsLds && !fast2;
     sh.hbmVisited = !fast && !fast2 && capV >= kVisitedHbmMinCap;
     const int fastKind = fast ? (sh.maxch == 2 ? 16 : 0) + fastMode(g->n, g->G)
-                       : fast2 ? 8 + fastMode(g->n, g->G) : sh.hbmVisited ? -1 : 0;
+                       : fast2 ? (sh.maxch == 2 ? 20 : 8) + fastMode(g->n, g->G) : sh.hbmVisited ? -1 : 0;
     const FlatLds lay = fast ? flatFastLdsLayout(capV)
                       : fast2 ? flatFast2LdsLayout(capV)
                               : flatLdsLayout(Ls, sh.maxch, sh.hbmVisited ? 0 : capV, sh.ldsFilter ? sh.fbits : 0);

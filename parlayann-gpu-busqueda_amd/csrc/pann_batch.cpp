@@ -151,20 +151,20 @@ int planFlatPart(pann_batch* b, int LsLo, int LsHi, bool top, BatchPart* P, int*
     const int fbLds = std::min(c.lds_filter_bits ? c.lds_filter_bits : kFilterBitsLds, fbMax);
     const uint32_t derived = flatVisitedCap(LsHi);
     const uint32_t capV = !c.visited_slots ? derived : top ? c.visited_slots : std::min(c.visited_slots, derived);
-    // one frontier key per lane (filter in LDS; rows of up to 128 entries in two halves),
-    // or two (64 < Ls <= 128, rows of at most 64, filter in HBM)
+    // one frontier key per lane (filter in LDS), or two (64 < Ls <= 128, filter in HBM);
+    // rows of 65 to 128 entries are read in two halves
     const bool fast = b->maxch <= 2 && LsHi <= kFastLsMax && fbLds >= kFastFilterBits;
-    const bool row2 = fast && b->maxch == 2;
-    const bool fast2 = b->maxch == 1 && !fast && LsHi <= kFast2LsMax;
+    const bool row2 = b->maxch == 2;
+    const bool fast2 = b->maxch <= 2 && !fast && LsHi <= kFast2LsMax;
     const bool hbmVisited = !fast && !fast2 && capV >= kVisitedHbmMinCap;
     // a general launch whose shortest beam already needs a larger filter keeps none in LDS
     const int fbLaunch = (!fast && !fast2 && flatFilterBits(LsLo + 1) > fbLds) ? 0 : fbLds;
     const int mode = fastMode(b->graph->n, b->graph->G);
     P->LsLo = LsLo;
     P->LsHi = LsHi;
-    P->fastKind = row2 ? 16 + mode
+    P->fastKind = fast && row2 ? 16 + mode
                 : fast ? (c.prefetch == 2 ? 12 + mode : mode + (c.prefetch ? 2 : 0))
-                       : fast2 ? 8 + mode : hbmVisited ? -1 : 0;
+                       : fast2 ? (row2 ? 20 : 8) + mode : hbmVisited ? -1 : 0;
     // several wavefronts per query: the general form only
     const int wpq = (!fast && !fast2 && c.waves_per_query > 1) ? (int)c.waves_per_query : 1;
     FlatLds lay = fast ? flatFastLdsLayout(capV)

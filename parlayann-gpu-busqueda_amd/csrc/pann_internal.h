@@ -169,7 +169,8 @@ struct FlatLaunch {
                                // rows of 65 to 128 entries read in two halves (the launcher
                                // adds 2).  Two-register
                                // form (flatFast2LdsLayout, filter in gfilt): 9, or 10 below
-                               // 4 GiB (the launcher adds 2 for normsInRow).  0: general;
+                               // 4 GiB (the launcher adds 2 for normsInRow); 21 or 22 over
+                               // rows read in two halves.  0: general;
                                // -1: general with the visited table in HBM (gvkey/gvcnt)
     int32_t wpq;               // general form: wavefronts per query (0 or 1: one; 2 or 4:
                                // the cooperative kernel, one query per block, scratch per

@@ -93,14 +93,14 @@ def test_reentering_expanded_nodes(gpu, max_degree):
     """A cut below 1 keeps the frontier short, so nodes expanded earlier drop out of it and
     come back as candidates.  One that comes back as the later of two row entries sharing a
     seen-filter slot, whose slot held its id before the row, must still carry its expansion
-    count (the oracle's unvisited set excludes it); rows of up to 64 entries and rows read
-    in two halves."""
+    count (the oracle's unvisited set excludes it); one and two frontier registers (Ls up
+    to 64, up to 128), rows of up to 64 entries and rows read in two halves."""
     from pann import datagen
     rng = np.random.default_rng(100 + max_degree)
     n = 3000
     base, queries = datagen.gmm_dataset(n, 400, dim=100, ncomp=6, seeds=(41, 42, 43))
     graph = random_graph(n, max_degree, rng, dup_frac=0.1, min_degree=max_degree // 2)
-    for Ls in (20, 30, 50):
+    for Ls in (20, 30, 50, 90, 120):
         for cut in (0.8, 0.9):
             r = _run(base, graph, queries, Ls, 10, cut=cut)
             o = _oracle(base, graph, queries, Ls, 10, cut=cut)
