@@ -213,11 +213,11 @@ typedef struct pann_batch_config {
                                 keep no such table and ignore it (reads back 0) */
     uint32_t prefetch;       /* next-row prefetch depth, 0, 1 or 2: at the start of a hop, load
                                 the rows of the first (and second) unvisited frontier entries
-                                after the expanded node, the likely next nodes (flat graphs,
-                                max_Ls <= 64) */
+                                after the expanded node, the likely next nodes (flat graphs
+                                with rows of at most 64 entries, max_Ls <= 64) */
     uint32_t waves_per_query; /* wavefronts sharing one query, 1, 2 or 4 (0: 1), in the flat
                                  launches of the general form (beams above 128 or rows above
-                                 64 entries): the leader filters and merges the candidates,
+                                 128 entries): the leader filters and merges the candidates,
                                  all compute distances and move frontier chunks; the outputs
                                  do not depend on it (one query per block of that many) */
 } pann_batch_config;

@@ -599,10 +599,10 @@ int pann_batch_autotune(const pann_vectors* base, const pann_graph* g, const pan
             for (uint32_t vs : slotChoices) {
                 if (!h && (vs < 256u || vs > 16384 || vs < 2u * (uint32_t)max_Ls + 64)) continue;
                 for (uint32_t pf : {0u, 1u, 2u}) {
-                    if (pf && (h || max_Ls > kFastLsMax)) continue;
+                    if (pf && (h || max_Ls > kFastLsMax || g->G > 64)) continue;
                     // several wavefronts per query: flat graphs whose top class is the
-                    // general form (beams above 128 or rows above 64)
-                    const bool general = !h && (max_Ls > kFast2LsMax || g->G > 64);
+                    // general form (beams above 128 or rows above 128)
+                    const bool general = !h && (max_Ls > kFast2LsMax || g->G > 128);
                     for (uint32_t wq : {1u, 2u, 4u}) {
                         if (wq > 1 && (!general || wpb != 1)) continue;
                         cands.push_back({max_Ls, max_k, wpb, fb, vs, pf, wq});
