@@ -842,20 +842,6 @@ __device__ __forceinline__ const T* rowAt(const void* base, uint32_t i, uint32_t
     else return reinterpret_cast<const T*>(b + ((size_t)i * pitch + extra));
 }
 
-// Value of lane 8 * (lane / 8) + r (ds_swizzle bit mode: and 0x18, or r).
-__device__ __forceinline__ uint32_t groupBroadcast8(uint32_t x, int r) {
-    switch (r) {
-        case 0: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (0 << 5));
-        case 1: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (1 << 5));
-        case 2: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (2 << 5));
-        case 3: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (3 << 5));
-        case 4: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (4 << 5));
-        case 5: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (5 << 5));
-        case 6: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (6 << 5));
-        default: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (7 << 5));
-    }
-}
-
 template <bool DESC>
 __device__ __forceinline__ uint64_t sortWave(uint64_t x, int lane) {
 #pragma unroll

@@ -14,7 +14,9 @@
 // (element e = r*64 + lane), so an insertion position is one ballot popcount and an
 // insertion a whole-wave DPP lane shift; is_inw is not stored but read off W (see the
 // acceptance in expandNode); the sequential acceptance only visits neighbours that can
-// still pass the threshold, found by ballot.
+// still pass the threshold, found by ballot.  Rows of at most 64 entries keep entry e on
+// lane e and issue the vector loads of every entry that may pass the seen filter before
+// the filter's shared-slot resolution, so the filter and the gathers overlap.
 #include "device_common.h"
 
 namespace pann {
@@ -96,39 +98,40 @@ __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* fi
         const uint32_t j = ch * 64 + lane;
         a[ch] = (ch < (int)nch && j < width) ? row[j] : kEmpty;
     }
-    uint32_t nP = 0;
-#pragma unroll
-    for (int ch = 0; ch < MAXCH; ch++) {
-        if (ch >= (int)nch) break;
-        const bool part = a[ch] != kEmpty;
-        st.edges += __popcll(ballot(part));
-        const bool keep = seenFilterChunk<G>(filt, tag, fmask, a[ch], part, lane, lanesBelow);
-        const uint64_t km = ballot(keep);
-        if (keep) pend[nP + mbcnt(km)] = a[ch];
-        nP += (uint32_t)__popcll(km);
-        __builtin_amdgcn_wave_barrier();
-    }
-    st.comps += (int)nP;
-
-    // distances, then move candidate j to lane j of block j/64
     int dist[MAXCH];
     uint32_t id[MAXCH];
     bool valid[MAXCH];
-#pragma unroll
-    for (int b = 0; b < MAXCH; b++) {
-        valid[b] = false;
-        dist[b] = 0;
-        id[b] = kEmpty;
-        if ((uint32_t)b * 64 >= nP) continue;
-        const uint32_t nb = min(64u, nP - (uint32_t)b * 64);
-        const uint32_t rounds = (nb + 7) / 8;
+    if constexpr (MAXCH == 1) {
+        // Rows of at most 64 entries: entry e stays on lane e.  Every entry that may pass
+        // the seen filter has its vector load issued once the slots are read (group g loads
+        // entry 8g + r in round r), and the distance of entry e lands on lane e; the entries
+        // kept although not speculated (an earlier lane took their slot) load afterwards.
+        const uint32_t a0 = a[0];
+        const bool part = a0 != kEmpty;
+        st.edges += __popcll(ballot(part));
+        const SeenProbe sp = seenFilterBegin<G>(filt, tag, fmask, a0, part, lane);
+        const bool spec = part && (sp.coll || sp.prev != a0);
+        const uint32_t gmask = (uint32_t)(ballot(spec) >> (8 * grp)) & 0xFFu;
+        // rounds of entries that are not loaded keep whatever the registers hold: their
+        // distances are computed and thrown away.  The ids are broadcast again for the late
+        // loads rather than held (eight registers more would spill at the occupancy bound).
         int4 v[8];
 #pragma unroll
         for (int r = 0; r < 8; r++) {
-            v[r] = make_int4(0, 0, 0, 0);
-            const uint32_t i = (uint32_t)r * 8 + grp;
-            if ((uint32_t)r < rounds && i < nb)
-                v[r] = *reinterpret_cast<const int4*>(L.vec + (size_t)pend[b * 64 + i] * kVecBytes + sub * 16);
+            asm("" : "=v"(v[r].x), "=v"(v[r].y), "=v"(v[r].z), "=v"(v[r].w));
+            if ((gmask >> r) & 1u)
+                v[r] = *reinterpret_cast<const int4*>(L.vec + (size_t)groupBroadcast8(a0, r) * kVecBytes + sub * 16);
+        }
+        const bool keep = seenFilterEnd<G>(filt, sp, a0, part, lane, lanesBelow);
+        const uint64_t keepM = ballot(keep);
+        st.comps += __popcll(keepM);
+        const uint64_t late = keepM & ~ballot(spec);
+        if (late) {
+            const uint32_t lmask = (uint32_t)(late >> (8 * grp)) & 0xFFu;
+#pragma unroll
+            for (int r = 0; r < 8; r++)
+                if ((lmask >> r) & 1u)
+                    v[r] = *reinterpret_cast<const int4*>(L.vec + (size_t)groupBroadcast8(a0, r) * kVecBytes + sub * 16);
         }
         int part8[8];
         const int normMask = (L.normsInRow && sub == 7) ? -1 : 0;
@@ -137,14 +140,59 @@ __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* fi
             v[r].w &= ~normMask;
             part8[r] = slicePartial(v[r], qv);
         }
-        const int tot = transposeReduce8(part8, lane);
-        // lane l holds candidate (l & 7) * 8 + (l >> 3); the map is its own inverse
-        const int mine = __shfl(tot, (lane & 7) * 8 + (lane >> 3), 64);
-        if ((uint32_t)lane < nb) {
-            valid[b] = true;
-            dist[b] = qq + mine;
-            id[b] = pend[b * 64 + lane];
+        valid[0] = keep;
+        dist[0] = qq + transposeReduce8(part8, lane);  // distance of entry `lane`
+        id[0] = a0;
+    } else {
+        uint32_t nP = 0;
+#pragma unroll
+        for (int ch = 0; ch < MAXCH; ch++) {
+            if (ch >= (int)nch) break;
+            const bool part = a[ch] != kEmpty;
+            st.edges += __popcll(ballot(part));
+            const bool keep = seenFilterChunk<G>(filt, tag, fmask, a[ch], part, lane, lanesBelow);
+            const uint64_t km = ballot(keep);
+            if (keep) pend[nP + mbcnt(km)] = a[ch];
+            nP += (uint32_t)__popcll(km);
+            __builtin_amdgcn_wave_barrier();
         }
+        st.comps += (int)nP;
+
+
+        // distances, then move candidate j to lane j of block j/64
+#pragma unroll
+        for (int b = 0; b < MAXCH; b++) {
+            valid[b] = false;
+            dist[b] = 0;
+            id[b] = kEmpty;
+            if ((uint32_t)b * 64 >= nP) continue;
+            const uint32_t nb = min(64u, nP - (uint32_t)b * 64);
+            const uint32_t rounds = (nb + 7) / 8;
+            int4 v[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                v[r] = make_int4(0, 0, 0, 0);
+                const uint32_t i = (uint32_t)r * 8 + grp;
+                if ((uint32_t)r < rounds && i < nb)
+                    v[r] = *reinterpret_cast<const int4*>(L.vec + (size_t)pend[b * 64 + i] * kVecBytes + sub * 16);
+            }
+            int part8[8];
+            const int normMask = (L.normsInRow && sub == 7) ? -1 : 0;
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                v[r].w &= ~normMask;
+                part8[r] = slicePartial(v[r], qv);
+            }
+            const int tot = transposeReduce8(part8, lane);
+            // lane l holds candidate (l & 7) * 8 + (l >> 3); the map is its own inverse
+            const int mine = __shfl(tot, (lane & 7) * 8 + (lane >> 3), 64);
+            if ((uint32_t)lane < nb) {
+                valid[b] = true;
+                dist[b] = qq + mine;
+                id[b] = pend[b * 64 + lane];
+            }
+        }
+
     }
 
     // In-order acceptance; only neighbours that can still pass are visited.  is_inw (the

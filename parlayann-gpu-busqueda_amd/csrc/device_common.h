@@ -249,6 +249,20 @@ __device__ __forceinline__ uint64_t nextLane64(uint64_t v, uint64_t last, int la
     return lane == 63 ? last : (((uint64_t)hi << 32) | lo);
 }
 
+// Value of lane 8 * (lane / 8) + r (ds_swizzle bit mode: and 0x18, or r).
+__device__ __forceinline__ uint32_t groupBroadcast8(uint32_t x, int r) {
+    switch (r) {
+        case 0: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (0 << 5));
+        case 1: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (1 << 5));
+        case 2: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (2 << 5));
+        case 3: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (3 << 5));
+        case 4: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (4 << 5));
+        case 5: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (5 << 5));
+        case 6: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (6 << 5));
+        default: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (7 << 5));
+    }
+}
+
 // One compare-exchange stage at distance j < 64 inside a register: the lane keeps the
 // smaller of itself and its partner iff keepMin (one 64-bit compare).
 __device__ __forceinline__ uint64_t cmpXchg(uint64_t x, int j, bool keepMin, int lane) {
