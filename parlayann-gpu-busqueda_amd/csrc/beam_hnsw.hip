@@ -25,7 +25,8 @@ namespace {
 using namespace dev;
 
 // Sorted u64 keys (d << 32 | id) spread over the wave: element e = r*64 + lane in v[r].
-template <int WR>
+// SKIP: operations touch only the registers that hold live elements (see below).
+template <int WR, bool SKIP>
 struct WaveList {
     uint64_t v[WR];
     int size;
@@ -42,22 +43,41 @@ struct WaveList {
             if (r == (e >> 6)) x = readlane64(v[r], e & 63);
         return x;
     }
+    // Elements at and past `size` are never read, so with SKIP every operation touches
+    // only the registers that hold live elements (size and pos are wave-uniform: the
+    // skips are scalar branches).  Lists of one or two registers gain nothing from the
+    // branches, and the kernels for rows wider than 64 entries run out of registers with
+    // them, so those shift and compare every register.
     __device__ int countLess(uint64_t key, int lane) const {
         int c = 0;
+        const int sz = __builtin_amdgcn_readfirstlane(size);
 #pragma unroll
-        for (int r = 0; r < WR; r++) c += __popcll(ballot(r * 64 + lane < size && v[r] < key));
+        for (int r = 0; r < WR; r++) {
+            if (SKIP && r * 64 >= sz) break;
+            c += __popcll(ballot(r * 64 + lane < size && v[r] < key));
+        }
         return c;
     }
     __device__ bool hasDist(uint32_t d, int lane) const {
         uint64_t m = 0;
+        const int sz = __builtin_amdgcn_readfirstlane(size);
 #pragma unroll
-        for (int r = 0; r < WR; r++) m |= ballot(r * 64 + lane < size && (uint32_t)(v[r] >> 32) == d);
+        for (int r = 0; r < WR; r++) {
+            if (SKIP && r * 64 >= sz) break;
+            m |= ballot(r * 64 + lane < size && (uint32_t)(v[r] >> 32) == d);
+        }
         return m != 0;
     }
-    // Lane shifts are whole-wave DPP moves (no LDS round trip).
+    // Lane shifts are whole-wave DPP moves (no LDS round trip).  With SKIP, registers
+    // wholly below pos keep their elements and those past the new last element (index
+    // size) hold none, so neither is shifted.
     __device__ void insertAt(int pos, uint64_t key, int lane) {
+        const int sz = __builtin_amdgcn_readfirstlane(size);
+        const int ps = __builtin_amdgcn_readfirstlane(pos);
 #pragma unroll
         for (int r = WR - 1; r >= 0; r--) {
+            if (SKIP && r * 64 > sz) continue;
+            if (SKIP && r * 64 + 63 < ps) break;
             const uint64_t first = r > 0 ? readlane64(v[r > 0 ? r - 1 : 0], 63) : ~0ull;
             const uint64_t prev = prevLane64(v[r], first, lane);
             const int e = r * 64 + lane;
@@ -66,14 +86,19 @@ struct WaveList {
         size++;
     }
     __device__ void popFront(int lane) {
+        const int sz = __builtin_amdgcn_readfirstlane(size);
 #pragma unroll
         for (int r = 0; r < WR; r++) {
+            if (SKIP && r * 64 >= sz) break;
             const uint64_t last = r + 1 < WR ? readlane64(v[r + 1 < WR ? r + 1 : r], 0) : ~0ull;
             v[r] = nextLane64(v[r], last, lane);
         }
         size--;
     }
 };
+
+template <int WR, int MAXCH>
+constexpr bool kListSkips = MAXCH == 1 && WR >= 3;
 
 struct LayerState {
     int comps, edges, hops, flags;
@@ -83,7 +108,7 @@ struct LayerState {
 template <int WR, int MAXCH, bool G>
 __device__ void expandNode(const HnswLaunch& L, unsigned char* lds, uint32_t* filt, uint32_t fmask,
                            uint32_t u, int layer, int ef, const int4& qv, int qq,
-                           WaveList<WR>& W, WaveList<WR>& cand, LayerState& st, int lane,
+                           WaveList<WR, kListSkips<WR, MAXCH>>& W, WaveList<WR, kListSkips<WR, MAXCH>>& cand, LayerState& st, int lane,
                            uint64_t lanesBelow) {
     const int grp = lane >> 3, sub = lane & 7;
     uint32_t* pend = reinterpret_cast<uint32_t*>(lds + L.lds.pend);
@@ -257,7 +282,7 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
 #pragma unroll
     for (int e = 0; e < 8; e++) eps[e] = L.entries[e];
     LayerState st{0, 0, 0, 0};
-    WaveList<WR> W, cand;
+    WaveList<WR, kListSkips<WR, MAXCH>> W, cand;
     W.clear();
     cand.clear();
 
@@ -341,16 +366,16 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
     __builtin_amdgcn_wave_barrier();
 }
 
-// On rows of at most 64 entries, one- and two-register lists are held to 128 VGPRs (four
-// wavefronts per SIMD; a few spills) and longer lists to 168 (three): the hop is a chain
-// of dependent memory round trips, and ef=25 / ef=100 / ef=256 / ef=350 on the 1M HNSW run
-// 1.11x / 1.17x / 1.3x / 1.2x faster than with the registers the compiler would take
-// (one-register lists held to 96 VGPRs run 1.15x slower); eight-register lists (191 VGPRs
-// unbounded, two wavefronts) run ef=500 in 34.7 ms instead of 41.7 at 168 with 96 bytes
-// of spills.
+// On rows of at most 64 entries, one-register lists are held to 96 VGPRs (five wavefronts
+// per SIMD) and longer lists to 128 (four): the hop is a chain of dependent memory round
+// trips that more wavefronts hide.  With the query parameters in scalar registers the
+// kernels need 87 to 126 VGPRs up to six registers, so only the seven- and eight-register
+// lists spill at 128 (24 and 64 bytes beyond the entry-point array), and the 1M HNSW runs
+// ef=400 in 20.5 ms instead of 24.5 at 168 VGPRs (three wavefronts), ef=500 in 29.9
+// instead of 30.3.
 template <int WR, int MAXCH>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(MAXCH == 1 && WR <= 2 ? 4 : MAXCH == 1 ? 3 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(MAXCH == 1 ? (WR == 1 ? 5 : 4) : 1, 8)))
 void beamHnswKernel(HnswLaunch L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = laneId();
@@ -370,6 +395,17 @@ void beamHnswKernel(HnswLaunch L) {
             Ls = L.qp[q].Ls;
             k = L.qp[q].k;
             visit = L.qp[q].visit_limit;
+        }
+        // q and the parameters are the same in every lane.  Read through generic pointers
+        // they would count as per-lane values, and with them ef, the list sizes and every
+        // branch on those: saying so keeps them in scalar registers and the branches scalar.
+        q = (uint32_t)__builtin_amdgcn_readfirstlane((int)q);
+        Ls = __builtin_amdgcn_readfirstlane(Ls);
+        k = __builtin_amdgcn_readfirstlane(k);
+        {
+            const uint64_t v = (uint64_t)visit;
+            visit = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32)) << 32) |
+                              (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v));
         }
         const int fb = hnswFilterBits(Ls);
         if (Ls < 1 || Ls > L.LsMax || k < 0 || (uint32_t)k > L.outStride ||
