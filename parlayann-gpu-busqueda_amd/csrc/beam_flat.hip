@@ -11,7 +11,8 @@
 //     with a byte tag table and resolved with a scalar loop over the (rare) groups;
 //   * distances: 8 lanes x 16 B per 128-B row, 8 rows per load instruction,
 //     v_dot4_i32_i8 for q.p and p.p (no norm array), DPP transpose-reduce;
-//   * candidates: in-register bitonic sort of (d << 32 | id) keys;
+//   * candidates: in-register bitonic sort of (d << 32 | id) keys, or a rank sort when
+//     at most kRankSortMax are left (the usual case of long beams);
 //   * merge: every merged position computes its source with a ballot rank (no shifts).
 #include "device_common.h"
 
@@ -132,6 +133,29 @@ __device__ __forceinline__ uint64_t frontierOfCountKey(uint64_t k) {
 
 // VGLOBAL: the visited table lives in this wave's HBM region (gvk / gvc) rather than in
 // LDS, for beams whose table would otherwise bound how many wavefronts share a CU.
+// Up to this many candidates of the general form are sorted by rank instead of by the
+// bitonic network.
+constexpr int kRankSortMax = 16;
+
+// Ascending sort of one key per lane when the valid keys (the lanes of vm) are few and
+// the others hold ~0: a valid key's rank counts the smaller keys and the equal ones on
+// lower lanes, the others follow in lane order; a forward permute of each half moves the
+// key of rank r to lane r.
+__device__ __forceinline__ uint64_t rankSortFew(uint64_t key, uint64_t vm, int lane, uint64_t lanesBelow) {
+    int rank = 0;
+    uint64_t m = vm;
+    while (m) {
+        const int j = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        const uint64_t kj = readlane64(key, j);
+        rank += (kj < key || (kj == key && j < lane)) ? 1 : 0;
+    }
+    if (!((vm >> lane) & 1ull)) rank = __popcll(vm) + __popcll(~vm & lanesBelow);
+    const int lo = __builtin_amdgcn_ds_permute(rank << 2, (int)(uint32_t)key);
+    const int hi = __builtin_amdgcn_ds_permute(rank << 2, (int)(uint32_t)(key >> 32));
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
 template <int MAXCH, bool FGLOBAL, bool VGLOBAL>
 __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, const QueryCfg& c,
                           uint32_t* gfilt, uint32_t* gvk, unsigned char* gvc, Prof& P) {
@@ -195,96 +219,173 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
     P.mark(kProfInit);
     P.count(kProfQueries);
 
+    uint32_t predId = kEmpty, nextPred = kEmpty;  // one-chunk rows: the node whose row was
+    uint32_t aNext = kEmpty;                        // prefetched into aNext, the next guess
     while (u >= 0 && visits < c.limit) {
         const uint64_t ukey = F[u];
         const uint32_t uid = (uint32_t)ukey;
-
-        // ---- adjacency row, one coalesced dword per lane per 64 entries ----
-        uint32_t a[MAXCH];
-        const uint32_t* row = L.adj + (size_t)uid * G;
-#pragma unroll
-        for (int ch = 0; ch < MAXCH; ch++) {
-            const uint32_t j = ch * 64 + lane;
-            a[ch] = (ch < (int)nch && j < G) ? row[j] : kEmpty;
-        }
-
-        // ---- visit u: frontier flag and visited multiset count ----
-        bool overflow = false;
-        if (lane == 0) {
-            FV[u] = 1;
-            overflow = !visitedInsert<VGLOBAL>(vkey, vcnt, uid, logCap, vmask, capV, distinctVisited);
-        }
-        distinctVisited = (uint32_t)readlane((int)distinctVisited, 0);
-        if (readlane((int)overflow, 0)) {
-            flags |= kFlagOverflow;
-            break;
-        }
-        visits++;
-        hops++;
-        P.mark(kProfVisit);
-        P.count(kProfHops);
-
-        const int cutoff = (fsize < Ls) ? 0x7FFFFFFF : (int)(F[fsize - 1] >> 32);
-
-        // ---- lossy seen filter in row order (ParlayANN has_been_seen) ----
-        uint32_t nP = 0;
-#pragma unroll
-        for (int ch = 0; ch < MAXCH; ch++) {
-            if (ch >= (int)nch) break;
-            const uint32_t ai = a[ch];
-            edges += __popcll(ballot(ai != kEmpty));
-            const bool part = (ai != kEmpty) && (ai != c.qid);
-            const bool keep = seenFilterChunk<FGLOBAL>(filt, tag, fmask, ai, part, lane, lanesBelow);
-            const uint64_t km = ballot(keep);
-            if (keep) pend[nP + mbcnt(km)] = ai;
-            nP += (uint32_t)__popcll(km);
-            __builtin_amdgcn_wave_barrier();
-        }
-        comps += (int)nP;
-        P.mark(kProfFilter);
-
-        // ---- distances: 8 rows per round, 8 lanes x 16 B each ----
         uint64_t key[MAXCH];
-#pragma unroll
-        for (int b = 0; b < MAXCH; b++) {
-            key[b] = ~0ull;
-            if ((uint32_t)b * 64 >= nP) continue;
-            const uint32_t nb = min(64u, nP - (uint32_t)b * 64);
-            const uint32_t rounds = (nb + 7) / 8;
+        int cutoff;
+
+        if constexpr (MAXCH == 1) {
+            // ---- rows of at most 64 entries: entry e stays on lane e.  The row comes from
+            // the prefetch when u is the node it guessed; the seen filter's slot reads are
+            // issued before the wait on u's visit, so the two round trips overlap; every
+            // entry that may pass the filter has its vector load issued once the slots are
+            // read, the rest of the kept entries (an earlier lane took their slot) after ----
+            const uint32_t a0 = uid == predId ? aNext
+                                              : ((uint32_t)lane < G ? L.adj[(size_t)uid * G + lane] : kEmpty);
+            // the row of the entry the previous scan found after u: the next node unless
+            // this hop brings a closer candidate.  A wrong guess costs one row.
+            predId = nextPred;
+            aNext = (predId != kEmpty && (uint32_t)lane < G) ? L.adj[(size_t)predId * G + lane] : kEmpty;
+            const bool part = (a0 != kEmpty) && (a0 != c.qid);
+            const SeenProbe sp = seenFilterBegin<FGLOBAL>(filt, tag, fmask, a0, part, lane);
+
+            bool overflow = false;
+            if (lane == 0) {
+                FV[u] = 1;
+                overflow = !visitedInsert<VGLOBAL>(vkey, vcnt, uid, logCap, vmask, capV, distinctVisited);
+            }
+            distinctVisited = (uint32_t)readlane((int)distinctVisited, 0);
+            if (readlane((int)overflow, 0)) {
+                flags |= kFlagOverflow;
+                break;
+            }
+            visits++;
+            hops++;
+            P.mark(kProfVisit);
+            P.count(kProfHops);
+            cutoff = (fsize < Ls) ? 0x7FFFFFFF : (int)(F[fsize - 1] >> 32);
+            edges += __popcll(ballot(a0 != kEmpty));
+
+            const bool spec = part && (sp.coll || sp.prev != a0);
+            const uint32_t gmask = (uint32_t)(ballot(spec) >> (8 * grp)) & 0xFFu;
+            // rounds of entries that are not loaded keep whatever the registers hold: their
+            // distances are computed and thrown away
             int4 v[8];
 #pragma unroll
             for (int r = 0; r < 8; r++) {
-                v[r] = make_int4(0, 0, 0, 0);
-                const uint32_t i = (uint32_t)r * 8 + grp;
-                if ((uint32_t)r < rounds && i < nb) {
-                    const uint32_t id = pend[b * 64 + i];
-                    v[r] = *reinterpret_cast<const int4*>(L.base + (size_t)id * kVecBytes + sub * 16);
-                }
+                asm("" : "=v"(v[r].x), "=v"(v[r].y), "=v"(v[r].z), "=v"(v[r].w));
+                if ((gmask >> r) & 1u)
+                    v[r] = *reinterpret_cast<const int4*>(L.base + (size_t)groupBroadcast8(a0, r) * kVecBytes + sub * 16);
             }
-            // expansion count of the candidate this lane ends up holding, read while the
-            // vectors are in flight
-            const uint32_t il = (uint32_t)sub * 8 + grp;
-            uint32_t cid = 0, cnt = 0;
-            if (il < nb) {
-                cid = pend[b * 64 + il];
-                cnt = visitedCount<VGLOBAL>(vkey, vcnt, cid, logCap, vmask);
+            const bool keep = seenFilterEnd<FGLOBAL>(filt, sp, a0, part, lane, lanesBelow);
+            const uint64_t keepM = ballot(keep);
+            comps += __popcll(keepM);
+            const uint64_t late = keepM & ~ballot(spec);
+            if (late) {
+                const uint32_t lmask = (uint32_t)(late >> (8 * grp)) & 0xFFu;
+#pragma unroll
+                for (int r = 0; r < 8; r++)
+                    if ((lmask >> r) & 1u)
+                        v[r] = *reinterpret_cast<const int4*>(L.base + (size_t)groupBroadcast8(a0, r) * kVecBytes + sub * 16);
             }
+            P.mark(kProfFilter);
+            // expansion count of each kept entry, read while the vectors are in flight
+            const uint32_t cnt = keep ? visitedCount<VGLOBAL>(vkey, vcnt, a0, logCap, vmask) : 0u;
             int part8[8];
 #pragma unroll
             for (int r = 0; r < 8; r++) {
                 v[r].w &= ~normMask;
                 part8[r] = slicePartial(v[r], qv);
             }
-            const int tot = transposeReduce8(part8, lane);
-            if (il < nb) {
-                const int d = qq + tot;
-                if (d < cutoff) key[b] = countKey(d, cid, cnt);
+            const int d = qq + transposeReduce8(part8, lane);  // distance of entry `lane`
+            key[0] = (keep && d < cutoff) ? countKey(d, a0, cnt) : ~0ull;
+        } else {
+            // ---- adjacency row, one coalesced dword per lane per 64 entries ----
+            uint32_t a[MAXCH];
+            const uint32_t* row = L.adj + (size_t)uid * G;
+#pragma unroll
+            for (int ch = 0; ch < MAXCH; ch++) {
+                const uint32_t j = ch * 64 + lane;
+                a[ch] = (ch < (int)nch && j < G) ? row[j] : kEmpty;
+            }
+
+            // ---- visit u: frontier flag and visited multiset count ----
+            bool overflow = false;
+            if (lane == 0) {
+                FV[u] = 1;
+                overflow = !visitedInsert<VGLOBAL>(vkey, vcnt, uid, logCap, vmask, capV, distinctVisited);
+            }
+            distinctVisited = (uint32_t)readlane((int)distinctVisited, 0);
+            if (readlane((int)overflow, 0)) {
+                flags |= kFlagOverflow;
+                break;
+            }
+            visits++;
+            hops++;
+            P.mark(kProfVisit);
+            P.count(kProfHops);
+
+            cutoff = (fsize < Ls) ? 0x7FFFFFFF : (int)(F[fsize - 1] >> 32);
+
+            // ---- lossy seen filter in row order (ParlayANN has_been_seen) ----
+            uint32_t nP = 0;
+#pragma unroll
+            for (int ch = 0; ch < MAXCH; ch++) {
+                if (ch >= (int)nch) break;
+                const uint32_t ai = a[ch];
+                edges += __popcll(ballot(ai != kEmpty));
+                const bool part = (ai != kEmpty) && (ai != c.qid);
+                const bool keep = seenFilterChunk<FGLOBAL>(filt, tag, fmask, ai, part, lane, lanesBelow);
+                const uint64_t km = ballot(keep);
+                if (keep) pend[nP + mbcnt(km)] = ai;
+                nP += (uint32_t)__popcll(km);
+                __builtin_amdgcn_wave_barrier();
+            }
+            comps += (int)nP;
+            P.mark(kProfFilter);
+
+            // ---- distances: 8 rows per round, 8 lanes x 16 B each ----
+#pragma unroll
+            for (int b = 0; b < MAXCH; b++) {
+                key[b] = ~0ull;
+                if ((uint32_t)b * 64 >= nP) continue;
+                const uint32_t nb = min(64u, nP - (uint32_t)b * 64);
+                const uint32_t rounds = (nb + 7) / 8;
+                int4 v[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    v[r] = make_int4(0, 0, 0, 0);
+                    const uint32_t i = (uint32_t)r * 8 + grp;
+                    if ((uint32_t)r < rounds && i < nb) {
+                        const uint32_t id = pend[b * 64 + i];
+                        v[r] = *reinterpret_cast<const int4*>(L.base + (size_t)id * kVecBytes + sub * 16);
+                    }
+                }
+                // expansion count of the candidate this lane ends up holding, read while the
+                // vectors are in flight
+                const uint32_t il = (uint32_t)sub * 8 + grp;
+                uint32_t cid = 0, cnt = 0;
+                if (il < nb) {
+                    cid = pend[b * 64 + il];
+                    cnt = visitedCount<VGLOBAL>(vkey, vcnt, cid, logCap, vmask);
+                }
+                int part8[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    v[r].w &= ~normMask;
+                    part8[r] = slicePartial(v[r], qv);
+                }
+                const int tot = transposeReduce8(part8, lane);
+                if (il < nb) {
+                    const int d = qq + tot;
+                    if (d < cutoff) key[b] = countKey(d, cid, cnt);
+                }
             }
         }
 
         P.mark(kProfDist);
-        // ---- sort candidates by (d, id) ----
-        bitonicSort<MAXCH>(key, lane);
+        // ---- sort candidates by (d, id); a few candidates (the usual case of long beams)
+        // by rank ----
+        if constexpr (MAXCH == 1) {
+            const uint64_t vm = ballot(key[0] != ~0ull);
+            if (__popcll(vm) <= kRankSortMax) key[0] = rankSortFew(key[0], vm, lane, lanesBelow);
+            else bitonicSort<1>(key, lane);
+        } else {
+            bitonicSort<MAXCH>(key, lane);
+        }
         P.mark(kProfSort);
 
         // ---- multiset union with the frontier ----
@@ -420,15 +521,20 @@ __device__ void searchOne(const FlatLaunch& L, unsigned char* lds, uint32_t q, c
         // ---- next: first frontier copy not visited.  Positions below both the first
         // entering copy and u + 1 hold old, visited copies ----
         u = -1;
+        int u2 = -1;  // the next unvisited copy after u in the same 64 positions
         for (int p0 = placed > 0 ? min(pmin, uPrev + 1) : uPrev + 1; p0 < fsize; p0 += 64) {
             const int p = p0 + lane;
             const bool nv = (p < fsize) && !FV[p];
             const uint64_t m = ballot(nv);
             if (m) {
                 u = p0 + __ffsll((unsigned long long)m) - 1;
+                const uint64_t m2 = m & (m - 1);
+                if (m2) u2 = p0 + __ffsll((unsigned long long)m2) - 1;
                 break;
             }
         }
+        if constexpr (MAXCH == 1)
+            nextPred = u2 >= 0 ? (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)F[u2]) : kEmpty;
         P.mark(kProfNext);
     }
 
@@ -1738,7 +1844,10 @@ __global__ __launch_bounds__(256) void beamFlatKernel(FlatLaunch L) {
 // Long beams: with the visited table in HBM a wavefront needs only its frontier and a few
 // small buffers in LDS, so up to four wavefronts share a SIMD (128 VGPRs; rows of more
 // than 64 entries need more and get three); each hop is a chain of dependent HBM round
-// trips and the wavefronts hide one another's.
+// trips and the wavefronts hide one another's.  On rows of at most 64 entries the chain
+// is shortened where it can be: the row of the guessed next node is prefetched, the seen
+// filter's slot reads overlap the visit, and the vector loads go out before the filter's
+// shared-slot resolution (five wavefronts per SIMD at 96 VGPRs spill and run 2% slower).
 template <int MAXCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MAXCH == 1 ? 4 : 3, 8))) void beamFlatWideKernel(FlatLaunch L) {
     drainQueries<MAXCH, -1>(L);
