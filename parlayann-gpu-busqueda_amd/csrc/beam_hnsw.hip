@@ -45,9 +45,11 @@ struct WaveList {
     }
     // Elements at and past `size` are never read, so with SKIP every operation touches
     // only the registers that hold live elements (size and pos are wave-uniform: the
-    // skips are scalar branches).  Lists of one or two registers gain nothing from the
-    // branches, and the kernels for rows wider than 64 entries run out of registers with
-    // them, so those shift and compare every register.
+    // skips are scalar branches).  Only lists of seven and eight registers gain from the
+    // branches (1M HNSW ef 256 / 350 ran 1-2% slower with them on five and six registers,
+    // ef 150 / 200 6% slower on three and four), and the kernels for rows wider than 64
+    // entries run out of registers with them, so the others shift and compare every
+    // register.
     __device__ int countLess(uint64_t key, int lane) const {
         int c = 0;
         const int sz = __builtin_amdgcn_readfirstlane(size);
@@ -98,7 +100,7 @@ struct WaveList {
 };
 
 template <int WR, int MAXCH>
-constexpr bool kListSkips = MAXCH == 1 && WR >= 3;
+constexpr bool kListSkips = MAXCH == 1 && WR >= 7;
 
 struct LayerState {
     int comps, edges, hops, flags;
@@ -366,16 +368,18 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
     __builtin_amdgcn_wave_barrier();
 }
 
-// On rows of at most 64 entries, one-register lists are held to 96 VGPRs (five wavefronts
-// per SIMD) and longer lists to 128 (four): the hop is a chain of dependent memory round
-// trips that more wavefronts hide.  With the query parameters in scalar registers the
-// kernels need 87 to 126 VGPRs up to six registers, so only the seven- and eight-register
-// lists spill at 128 (24 and 64 bytes beyond the entry-point array), and the 1M HNSW runs
-// ef=400 in 20.5 ms instead of 24.5 at 168 VGPRs (three wavefronts), ef=500 in 29.9
-// instead of 30.3.
+// On rows of at most 64 entries, lists of up to four registers are held to 96 VGPRs (five
+// wavefronts per SIMD) and longer lists to 128 (four): the hop is a chain of dependent
+// memory round trips that more wavefronts hide.  With the query parameters in scalar
+// registers the kernels need 87 to 126 VGPRs up to six registers; held to 96, lists of two
+// to four registers spill 8 / 24 / 48 bytes beyond the entry-point array and still run
+// the 1M HNSW 8-14% faster at ef 64-200 than at four wavefronts (five and six registers at
+// five wavefronts: 4-13% slower).  At 128 the seven- and eight-register lists spill 24 and
+// 64 bytes and run ef=400 in 20.5 ms instead of 24.5 at 168 VGPRs (three wavefronts),
+// ef=500 in 29.9 instead of 30.3.
 template <int WR, int MAXCH>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(MAXCH == 1 ? (WR == 1 ? 5 : 4) : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(MAXCH == 1 ? (WR <= 4 ? 5 : 4) : 1, 8)))
 void beamHnswKernel(HnswLaunch L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = laneId();
