@@ -368,8 +368,9 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
     __builtin_amdgcn_wave_barrier();
 }
 
-// On rows of at most 64 entries, lists of up to four registers are held to 96 VGPRs (five
-// wavefronts per SIMD) and longer lists to 128 (four): the hop is a chain of dependent
+// On rows of at most 64 entries, one-register lists are held to 80 VGPRs (six wavefronts
+// per SIMD, 16 bytes of spills; 3-7% faster at ef 10-50 than at five), lists of two to four
+// registers to 96 (five) and longer lists to 128 (four): the hop is a chain of dependent
 // memory round trips that more wavefronts hide.  With the query parameters in scalar
 // registers the kernels need 87 to 126 VGPRs up to six registers; held to 96, lists of two
 // to four registers spill 8 / 24 / 48 bytes beyond the entry-point array and still run
@@ -379,7 +380,7 @@ __device__ void hnswOne(const HnswLaunch& L, unsigned char* lds, uint32_t* gfilt
 // ef=500 in 29.9 instead of 30.3.
 template <int WR, int MAXCH>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(MAXCH == 1 ? (WR <= 4 ? 5 : 4) : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(MAXCH == 1 ? (WR == 1 ? 6 : WR <= 4 ? 5 : 4) : 1, 8)))
 void beamHnswKernel(HnswLaunch L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = laneId();

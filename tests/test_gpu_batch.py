@@ -75,7 +75,7 @@ def test_hnsw_batch_matches_oracle(gpu):
     rng = np.random.default_rng(2)
     b = search.Vectors(base)
     idx = search.HnswIndex(h, b)
-    nq = 6000  # above the resident wavefronts of every class (5,120 at most): one launch per class
+    nq = 7000  # above the resident wavefronts of every class (6,144 at most): one launch per class
     rows = np.ascontiguousarray(np.resize(queries, (nq, queries.shape[1])))
     batch = search.Batch(nq, 300, 100, hnsw=idx)
     assert batch.config()[1] < nq
