@@ -53,6 +53,16 @@ def test_built_hnsw_parity(gpu, built, Ls, k):
     _assert_same(_run(base, h, queries, Ls, k), _oracle(base, h, queries, Ls, k), k, f"Ls={Ls} k={k}")
 
 
+@pytest.mark.parametrize("Ls", [55, 56, 119, 120, 183, 247, 311, 375, 439, 503])
+def test_list_width_boundaries(gpu, built, Ls):
+    """ef at the last and first value of each list width (lists of 1..8 registers of 64
+    entries, ef + 9 slots): the fullest lists, whose insertions drop the element past the
+    last register, in every kernel form (occupancy bounds and register skips differ by
+    width)."""
+    base, queries, h = built
+    _assert_same(_run(base, h, queries, Ls, 10), _oracle(base, h, queries, Ls, 10), 10, f"Ls={Ls}")
+
+
 @pytest.mark.parametrize("Ls", [40, 100, 300])
 def test_repeat_launch(gpu, built, Ls):
     """Back-to-back launches of one prepared searcher (from ef 65 on they share a
